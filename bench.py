@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: 1920x1080 float32 Gaussian smoothing + five derivatives on one
+MI355X, fused into one launch per image.
+
+A step is one call of the fused kernel over one 1920x1080 image (sigma 3,
+lobe radius 16 for all three lobes, all six outputs).  Inputs and outputs
+live in HBM before the timed region starts; the step rotates through
+``--sets`` buffer sets (about 58 MB each) so that the 256 MB Infinity Cache
+cannot hold the working set.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpix/s & % HBM roofline (1080p Gauss+5 derivs); p50/p99 ms served stream"
+HBM_PEAK_GBS = 8000.0
+NROW, NCOL, SIGMA = 1080, 1920, 3.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(nrow: int, ncol: int, radii, in_size: int = 4, out_size: int = 4) -> int:
+    """Input read once plus each output's valid region written once."""
+    total = nrow * ncol * in_size
+    for rv, rh in radii:
+        total += (nrow - 2 * rv) * (ncol - 2 * rh) * out_size
+    return total
+
+
+def cpu_baseline(img, sigma: float, budget_s: float = 12.0):
+    """The oracle's FP32 restatement (H pass then V pass per output, FP32
+    intermediate, no FMA contraction) on the host cores, over whole images,
+    as many images as fit in about ``budget_s`` seconds (at least one)."""
+    import oracle
+    import numpy as np
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    lobes = oracle.gauss_lobes(sigma)
+    outs = {m: np.zeros(img.shape, np.float32) for m in oracle.METHOD_ORDER}
+
+    def one():
+        for m in oracle.METHOD_ORDER:
+            oracle.gauss(m, sigma, img, outs[m], lobes=lobes, nthreads=cores)
+
+    one()                                             # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        one()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or (n >= 3 and el * (n + 1) / n > 2 * budget_s):
+            break
+    gpix = img.size * n / el / 1e9
+    return {"value": round(gpix, 5), "unit": "Gpix/s", "cores": cores, "kind": "port",
+            "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--sets", type=int, default=10, help="rotating buffer sets (>= 5 defeats the 256 MB MALL)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(images shard across GPUs with no communication; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import tucuda_amd
+    from tucuda_amd import synth
+
+    dev = torch.device("cuda:0")
+    img = synth.edge_image(1, NROW, NCOL)
+    g = tucuda_amd.FIRGaussianConvolver2(SIGMA)
+    l0, l1, l2 = (x.size for x in tucuda_amd.gauss_lobes(SIGMA))
+    r = [l0 - 1, l1, l2 - 1]
+    radii = [(r[v], r[h]) for h, v in (tucuda_amd.METHOD_LOBES[m] for m in tucuda_amd.METHOD_ORDER)]
+    alg_bytes = algorithmic_bytes(NROW, NCOL, radii)
+
+    d_img = torch.from_numpy(img).to(dev)
+    sets = []
+    for s in range(args.sets):
+        ins = d_img.clone()
+        outs = [torch.empty((NROW, NCOL), dtype=torch.float32, device=dev) for _ in range(6)]
+        sets.append((ins, outs))
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream(dev)
+    for k in range(args.warmup):
+        ins, outs = sets[k % args.sets]
+        g.convolve6(ins, outs)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(args.steps):
+        ins, outs = sets[k % args.sets]
+        starts[k].record(stream)
+        g.convolve6(ins, outs)
+        ends[k].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_kern_ms = float(np.mean(kern_ms))
+    med_kern_ms = float(np.median(kern_ms))
+    ms_per_step = total_ms / args.steps
+    gpix = NROW * NCOL / (ms_per_step * 1e-3) / 1e9
+    achieved = alg_bytes / (avg_kern_ms * 1e-3) / 1e9
+    log(f"kernel avg {avg_kern_ms*1e3:.2f} us median {med_kern_ms*1e3:.2f} us; "
+        f"wall per step {ms_per_step*1e3:.2f} us; {achieved:.0f} GB/s algorithmic")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "gauss6_1080p_f32_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": round(gpix, 4),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform [0,255] + 8 step edges)",
+        "config": {
+            "workload": "1920x1080 float32 image, sigma=3 (lobes 17/16/17, radius 16), "
+                        "smooth+diffH+diffV+diffHH+diffHV+diffVV fused in one launch",
+            "nrow": NROW, "ncol": NCOL, "sigma": SIGMA, "outputs": 6,
+            "buffer_sets": args.sets, "shift": True,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel_us_avg": round(avg_kern_ms * 1e3, 3),
+            "algorithmic_bytes": alg_bytes,
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        line["cpu_baseline"] = cpu_baseline(img, SIGMA, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+/*
+ * tucu.h -- C ABI of the MI355X (gfx950) separable FIR / Gaussian-derivative
+ * library libtucuda_amd.so.
+ *
+ * Plain C: pointers, sizes and POD descriptors only.  Every entry point
+ * returns a tucu_status; on failure tucu_last_error() returns a message for
+ * the calling thread.  Work is enqueued on the HIP stream passed as `stream`
+ * (a hipStream_t, NULL = the default stream) and is asynchronous to the
+ * host, as the reference's default-stream launches are.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the TUCuda tree).
+ */
+#ifndef TUCU_H
+#define TUCU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TUCU_API __attribute__((visibility("default")))
+
+typedef enum {
+    TUCU_OK = 0,
+    TUCU_ERR_INVALID = 1,          /* bad argument (null pointer, negative size, output too small) */
+    TUCU_ERR_LOBE_TOO_LARGE = 2,   /* FIRFilter.h:250-251 runtime_error("... too large lobe size!") */
+    TUCU_ERR_UNSUPPORTED_LOBE = 3, /* FIRFilter.h:343-344, 374-375 runtime_error("... unsupported ... lobe size") */
+    TUCU_ERR_BAD_ALLOC = 4,        /* allocator.h:83-84 std::bad_alloc */
+    TUCU_ERR_DEVICE = 5            /* a HIP runtime error (algorithm.h:85-94 prints and exits instead) */
+} tucu_status;
+
+typedef enum {
+    TUCU_U8 = 0,    /* u_char */
+    TUCU_F32 = 1,   /* float */
+    TUCU_F16 = 2,   /* IEEE binary16 storage, converted to float on load (extension) */
+    TUCU_BF16 = 3   /* bfloat16 storage, converted to float on load (extension) */
+} tucu_dtype;
+
+/* A 2-D image in device memory: the kernel-visible form of
+ * range<range_iterator<device_ptr<T>>> (iterator.h:385-434, 439-527):
+ * row-0 pointer, row stride in elements, rows, columns, element type. */
+typedef struct {
+    void*   data;
+    int64_t stride;
+    int32_t nrow;
+    int32_t ncol;
+    int32_t dtype;
+    int32_t reserved;
+} tucu_image;
+
+/* Output bits of tucu_gauss2_convolve, in FIRGaussianConvolver2's method order. */
+enum {
+    TUCU_SMOOTH = 1u << 0,  /* FIRGaussianConvolver.h:178-182 */
+    TUCU_DIFFH  = 1u << 1,  /* :190-194 */
+    TUCU_DIFFV  = 1u << 2,  /* :202-206 */
+    TUCU_DIFFHH = 1u << 3,  /* :214-218 */
+    TUCU_DIFFHV = 1u << 4,  /* :226-230 */
+    TUCU_DIFFVV = 1u << 5,  /* :238-242 */
+    TUCU_ALL6   = 0x3fu
+};
+
+/* Lobe rule for tucu_gauss2_lobes. */
+enum {
+    TUCU_RULE_REFERENCE = 0,   /* FIRGaussianConvolver.h:123-170 + FIRGaussianConvolver.cu:52-98 */
+    TUCU_RULE_CEIL4SIGMA = 1   /* extension: radius ceil(4 sigma) (at most 16), same normalisation */
+};
+
+/* Message describing the last failure on this thread ("" if none). */
+TUCU_API const char* tucu_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+TUCU_API int tucu_version(void);
+
+/* ---- device memory (cu::allocator / cu::Array2 storage) ---------------- */
+
+/* cu::allocator<T>::allocate (allocator.h:71-87): device memory, zero-filled,
+ * 256-byte aligned; bytes == 0 yields NULL.  Failure: TUCU_ERR_BAD_ALLOC. */
+TUCU_API int tucu_malloc(void** ptr, size_t bytes);
+
+/* cu::allocator<T>::deallocate (allocator.h:88-94); NULL is a no-op. */
+TUCU_API int tucu_free(void* ptr);
+
+/* Row stride (in elements) of a cu::Array2<T> row of `ncol` elements of
+ * `elem_size` bytes: rows padded to the allocator's 256-byte alignment
+ * (allocator.h:68, Array++.h:142). */
+TUCU_API size_t tucu_row_stride(size_t ncol, size_t elem_size);
+
+/* 2-D copy between host and device memory: the transfers behind
+ * cu::Array2<T>(host array) and Image<T>(cu::Array2<T>) (Testprog/gaussTest/
+ * main.cu:91, 108).  kind: 0 host->device, 1 device->host, 2 device->device.
+ * Pitches and width in bytes.  Synchronous, like the reference's copies. */
+TUCU_API int tucu_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                           size_t width_bytes, size_t height, int kind);
+
+/* hipDeviceSynchronize (the caller's cudaDeviceSynchronize(),
+ * Testprog/gaussTest/main.cu:95). */
+TUCU_API int tucu_synchronize(void);
+
+/* ---- FIRFilter2 --------------------------------------------------------- */
+
+/* FIRFilter2::initialize(lobeH, lobeV) size check (FIRFilter.h:245-261):
+ * TUCU_ERR_LOBE_TOO_LARGE when either length exceeds 17. */
+TUCU_API int tucu_fir2_check(int nH, int nV);
+
+/* FIRFilter2::initialize(lobeH, lobeV).convolve(in, ie, out, shift)
+ * (FIRFilter.h:245-261, 298-377; instantiated for u_char/float in and out,
+ * FIRFilter.cu:49-68).  Lobe lengths must be in {2,3,4,5,8,9,16,17}: odd
+ * lengths are symmetric half-kernels with lobe[len-1] at the centre, even
+ * lengths antisymmetric ones; lobe[0] is the outermost tap.  With
+ * rH = nH & ~1 and rV = nV & ~1, the output is written only on the
+ * (nrow-2rV) x (ncol-2rH) valid region, at offset (rV, rH) when shift != 0,
+ * else at (0, 0); nothing else of `out` is touched.  Images with
+ * nrow <= 2rV or ncol <= 2rH are a silent no-op (FIRFilter.h:304-312).
+ * in->dtype and out->dtype may be any tucu_dtype.  If out->nrow and
+ * out->ncol are nonzero they are checked against the region written.
+ * Coefficients travel as kernel arguments: no per-call upload. */
+TUCU_API int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
+                                const tucu_image* in, const tucu_image* out,
+                                int shift, void* stream);
+
+/* ---- FIRGaussianConvolver2 ---------------------------------------------- */
+
+/* detail::lobeSize(lobe, even) (FIRGaussianConvolver.cu:52-98) over a
+ * 17-element lobe. */
+TUCU_API int tucu_lobe_size(const float* lobe17, int even);
+
+/* FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)
+ * (FIRGaussianConvolver.h:123-170).  Each lobe buffer holds 17 floats; the
+ * lengths are returned in n0, n1, n2.  rule: TUCU_RULE_*. */
+TUCU_API int tucu_gauss2_lobes(float sigma, int rule,
+                               float* lobe0, int* n0, float* lobe1, int* n1,
+                               float* lobe2, int* n2);
+
+/* The six FIRGaussianConvolver2 methods (FIRGaussianConvolver.h:178-242)
+ * fused: one read of `in`, every output selected by `mask` (TUCU_SMOOTH ..
+ * TUCU_DIFFVV) written by the same launch.  outs[k] is the destination of
+ * method k; entries whose mask bit is clear are ignored.  Each output obeys
+ * the single-method contract of tucu_fir2_convolve with its own (H, V) lobe
+ * pair: smooth (l0,l0), diffH (l1,l0), diffV (l0,l1), diffHH (l2,l0),
+ * diffHV (l1,l1), diffVV (l0,l2).  The roles fix the parities: l0 and l2
+ * are symmetric lobes of length r+1 (centre last), l1 an antisymmetric lobe
+ * of length r, each radius r in 1..16.  Lobes of the reference rule
+ * ({3,5,9,17} and {2,4,8,16}) mean exactly what they mean to FIRFilter2;
+ * other radii serve TUCU_RULE_CEIL4SIGMA lobes. */
+TUCU_API int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
+                                  const float* lobe2, int n2,
+                                  const tucu_image* in, const tucu_image* outs,
+                                  unsigned mask, int shift, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TUCU_H */

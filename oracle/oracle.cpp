@@ -1,0 +1,321 @@
+// oracle.cpp -- CPU restatement of TUCuda's separable FIR / Gaussian-derivative
+// path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and the
+// CPU-baseline leg of the benchmark load this library as the checker.  The
+// product (libtucuda_amd.so) never links, loads or calls it.
+//
+// Two flavours are restated:
+//   * "f32": faithful to the reference arithmetic -- float coefficients made by
+//     the reference rule, an FP32 intermediate image, horizontal pass first,
+//     tap sums evaluated in source order with every product and sum rounded
+//     separately (this file is compiled with -ffp-contract=off).
+//   * "f64": the same algorithm in double precision (coefficients, conversion
+//     and accumulation), used as the precision reference.
+//
+// Reference files followed (paths relative to the TUCuda tree):
+//   TU/cu/FIRFilter.h:37-40      offsets / output sizes
+//   TU/cu/FIRFilter.h:66-176     paired tap sums, lobe[0] = outermost tap
+//   TU/cu/FIRFilter.h:181-231    buf[y][x] = window starting at x; valid-only writes
+//   TU/cu/FIRFilter.h:245-261    lobe length limit (17)
+//   TU/cu/FIRFilter.h:286-287    shifted output slice
+//   TU/cu/FIRFilter.h:298-377    early return, intermediate type, size dispatch
+//   TU/cu/FIRGaussianConvolver.h:123-170  Gaussian lobes and their normalisation
+//   TU/cu/FIRGaussianConvolver.h:178-242  (H, V) lobe pairs of the six methods
+//   FIRGaussianConvolver.cu:52-98         detail::lobeSize (1 % threshold, size classes)
+//   TU/cu/algorithm.h:119-144             element conversion on tile load
+//   TU/cu/fp16.h:55-80                    round-to-nearest-even half conversion
+//
+// Status codes: 0 ok, 2 lobe longer than 17, 3 unsupported lobe length,
+// 1 invalid argument.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstddef>
+#include <thread>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+enum { DT_U8 = 0, DT_F32 = 1, DT_F16 = 2, DT_BF16 = 3 };
+constexpr size_t LobeSizeMax = 17;          // FIRFilter.h:26
+
+// ---------------------------------------------------------------- conversions
+inline float half_to_float(uint16_t h)
+{
+    const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+    uint32_t exp  = (h >> 10) & 0x1f;
+    uint32_t mant = h & 0x3ffu;
+    uint32_t bits;
+    if (exp == 0) {
+        if (mant == 0) bits = sign;
+        else {                                  // subnormal: normalise
+            int e = -1;
+            do { ++e; mant <<= 1; } while (!(mant & 0x400u));
+            bits = sign | uint32_t(127 - 15 - e) << 23 | (mant & 0x3ffu) << 13;
+        }
+    } else if (exp == 0x1f) {
+        bits = sign | 0x7f800000u | mant << 13;
+    } else {
+        bits = sign | (exp + 127 - 15) << 23 | mant << 13;
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// Round-to-nearest-even float -> binary16 (the rounding fp16.h:60-61 asks the
+// F16C unit for).  Written from the IEEE-754 definition.
+inline uint16_t float_to_half(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return uint16_t(sign | 0x7e00u);          // NaN
+    if (ax >= 0x477ff000u) return uint16_t(sign | 0x7c00u);         // overflow -> inf
+    if (ax < 0x38800000u) {                                          // result subnormal or zero
+        // value = m * 2^(e-150); in units of the half subnormal step 2^-24
+        // that is m * 2^(e-126), so drop (126 - e) bits and round.
+        if (ax < 0x33000000u) return uint16_t(sign);                 // < 2^-25 rounds to 0
+        const uint32_t e = ax >> 23;
+        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        const int shift = 126 - int(e);                              // 14 .. 24
+        const uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1);
+        const uint32_t halfway = 1u << (shift - 1);
+        uint32_t r = q;
+        if (rem > halfway || (rem == halfway && (q & 1))) ++r;
+        return uint16_t(sign | r);
+    }
+    uint32_t h = ((ax >> 23) - 127 + 15) << 10 | ((ax >> 13) & 0x3ffu);
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1))) ++h;           // may carry into exponent
+    return uint16_t(sign | h);
+}
+
+inline float bf16_to_float(uint16_t b)
+{
+    const uint32_t bits = uint32_t(b) << 16;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+inline uint16_t float_to_bf16(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return uint16_t((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return uint16_t(x >> 16);
+}
+
+// float -> u_char store (FIRFilter.h:203, 229 store a float into a u_char
+// element).  Out-of-range values are undefined in C++; this restatement
+// defines the conversion as saturate to [0,255], then truncate toward zero,
+// with NaN -> 0.
+template <class A> inline uint8_t to_u8(A v)
+{
+    if (!(v > A(0))) return 0;
+    if (v >= A(255)) return 255;
+    return uint8_t(int(v));
+}
+
+template <class A> inline A load_px(const void* base, int dt, int64_t idx)
+{
+    switch (dt) {
+      case DT_U8:   return A(static_cast<const uint8_t*>(base)[idx]);
+      case DT_F32:  return A(static_cast<const float*>(base)[idx]);
+      case DT_F16:  return A(half_to_float(static_cast<const uint16_t*>(base)[idx]));
+      default:      return A(bf16_to_float(static_cast<const uint16_t*>(base)[idx]));
+    }
+}
+
+template <class A> inline void store_px(void* base, int dt, int64_t idx, A v)
+{
+    switch (dt) {
+      case DT_U8:   static_cast<uint8_t*>(base)[idx] = to_u8(v); break;
+      case DT_F32:  static_cast<float*>(base)[idx] = float(v); break;
+      case DT_F16:  static_cast<uint16_t*>(base)[idx] = float_to_half(float(v)); break;
+      default:      static_cast<uint16_t*>(base)[idx] = float_to_bf16(float(v)); break;
+    }
+}
+
+inline bool supported_lobe(int n)
+{
+    return n == 17 || n == 16 || n == 9 || n == 8 || n == 5 || n == 4 || n == 3 || n == 2;
+}
+
+// Paired tap sum of FIRFilter.h:66-176 over the window in(0) .. in(2r),
+// r = L & ~1.  Odd L: symmetric kernel, lobe[r] weights the centre.  Even L:
+// antisymmetric kernel (left minus right), no centre tap.  Evaluated left to
+// right as the source expression is.
+template <class A, class GET> inline A paired_sum(const GET& in, const A* lobe, int L)
+{
+    const int r = L & ~1;
+    A acc;
+    if (L & 1) {
+        acc = lobe[0] * (in(0) + in(2 * r));
+        for (int i = 1; i < r; ++i) acc = acc + lobe[i] * (in(i) + in(2 * r - i));
+        acc = acc + lobe[r] * in(r);
+    } else {
+        acc = lobe[0] * (in(0) - in(2 * r));
+        for (int i = 1; i < r; ++i) acc = acc + lobe[i] * (in(i) - in(2 * r - i));
+    }
+    return acc;
+}
+
+template <class F> void parallel_rows(int n, int nthreads, const F& f)
+{
+    if (nthreads <= 1 || n < 2 * nthreads) { f(0, n); return; }
+    std::vector<std::thread> th;
+    const int chunk = (n + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        const int b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// FIRFilter2::convolve (FIRFilter.h:298-377) in accumulate type A.
+template <class A>
+int fir2(const A* lobeH, int nH, const A* lobeV, int nV,
+         const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+         void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    if (nH < 0 || nV < 0 || nrow < 0 || ncol < 0) return 1;
+    if (size_t(nH) > LobeSizeMax || size_t(nV) > LobeSizeMax) return 2;   // :250-251
+    const int rV = nV & ~1, rH = nH & ~1;
+    if (nrow <= 2 * rV) return 0;                                          // :304-307
+    if (ncol <= 2 * rH) return 0;                                          // :309-312
+    if (!supported_lobe(nH) || !supported_lobe(nV)) return 3;              // :343-344, 374-375
+    const int ncb = ncol - 2 * rH;
+    std::vector<A> buf(size_t(nrow) * ncb);                                // _buf, T = float
+    // Horizontal pass (fir_filterH, :181-205): buf[y][x] over in[y][x .. x+2rH].
+    parallel_rows(nrow, nthreads, [&](int yb, int ye) {
+        std::vector<A> row(ncol);
+        for (int y = yb; y < ye; ++y) {
+            for (int x = 0; x < ncol; ++x) row[x] = load_px<A>(in, in_dt, int64_t(y) * in_stride + x);
+            A* b = &buf[size_t(y) * ncb];
+            for (int x = 0; x < ncb; ++x) {
+                const A* w = &row[x];
+                b[x] = paired_sum<A>([w](int i) { return w[i]; }, lobeH, nH);
+            }
+        }
+    });
+    // Vertical pass (fir_filterV, :207-231) into the shifted slice (:286-287).
+    const int oy = shift ? rV : 0, ox = shift ? rH : 0;
+    parallel_rows(nrow - 2 * rV, nthreads, [&](int yb, int ye) {
+        for (int y = yb; y < ye; ++y)
+            for (int x = 0; x < ncb; ++x) {
+                const A* c = &buf[size_t(y) * ncb + x];
+                const A v = paired_sum<A>([c, ncb](int i) { return c[size_t(i) * ncb]; }, lobeV, nV);
+                store_px<A>(out, out_dt, int64_t(y + oy) * out_stride + (x + ox), v);
+            }
+    });
+    return 0;
+}
+
+// detail::lobeSize (FIRGaussianConvolver.cu:52-98), generic in the element type.
+template <class A> size_t lobe_size(const A lobe[], bool even)
+{
+    const A epsilon = A(0.01);                  // .cu:58 (a float constant in the reference)
+    A th = 0;
+    for (size_t i = LobeSizeMax; i-- > 0; )
+        if (std::abs(lobe[i]) >= th) th = std::abs(lobe[i]);
+    th *= epsilon;
+    size_t siz;
+    for (siz = LobeSizeMax; siz-- > 0; )
+        if (std::abs(lobe[siz]) > th) { ++siz; break; }
+    if (even) {
+        if (siz <= 2) return 3;
+        else if (siz <= 5) return 5;
+        else if (siz <= 9) return 9;
+        else return 17;
+    } else {
+        if (siz <= 1) return 2;
+        else if (siz <= 4) return 4;
+        else if (siz <= 8) return 8;
+        else return 16;
+    }
+}
+
+// FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)
+// (FIRGaussianConvolver.h:123-170), generic in the element type.
+template <class A>
+void gauss_lobes(A sigma, A* lobe0, int* n0, A* lobe1, int* n1, A* lobe2, int* n2)
+{
+    A lb0[LobeSizeMax], lb1[LobeSizeMax], lb2[LobeSizeMax];
+    for (size_t i = 0; i < LobeSizeMax; ++i) {
+        const A dx = A(i) / sigma, dxdx = dx * dx;
+        lb0[i] = std::exp(A(-0.5) * dxdx);
+        lb1[i] = -dx * lb0[i];
+        lb2[i] = (dxdx - A(1)) * lb0[i];
+    }
+    const size_t s0 = lobe_size(lb0, true);
+    A sum = lb0[0];
+    for (size_t i = 1; i < s0; ++i) sum += (A(2) * lb0[i]);
+    for (size_t i = 0; i < s0; ++i) lobe0[i] = lb0[s0 - 1 - i] / std::abs(sum);
+
+    const size_t s1 = lobe_size(lb1, false);
+    sum = A(0);
+    for (size_t i = 0; i < s1; ++i) sum += (A(2) * A(i) * lb1[i]);
+    for (size_t i = 0; i < s1; ++i) lobe1[i] = lb1[s1 - i] / std::abs(sum);
+
+    const size_t s2 = lobe_size(lb2, true);
+    sum = A(0);
+    for (size_t i = 1; i < s2; ++i) sum += (A(i * i) * lb2[i]);
+    for (size_t i = 0; i < s2; ++i) lobe2[i] = lb2[s2 - 1 - i] / std::abs(sum);
+
+    *n0 = int(s0); *n1 = int(s1); *n2 = int(s2);
+}
+
+}   // namespace
+
+extern "C" {
+
+int orc_lobe_size_f32(const float* lobe17, int even)
+{
+    return int(lobe_size<float>(lobe17, even != 0));
+}
+
+int orc_lobe_size_f64(const double* lobe17, int even)
+{
+    return int(lobe_size<double>(lobe17, even != 0));
+}
+
+void orc_gauss_lobes_f32(float sigma, float* l0, int* n0, float* l1, int* n1, float* l2, int* n2)
+{
+    gauss_lobes<float>(sigma, l0, n0, l1, n1, l2, n2);
+}
+
+void orc_gauss_lobes_f64(double sigma, double* l0, int* n0, double* l1, int* n1, double* l2, int* n2)
+{
+    gauss_lobes<double>(sigma, l0, n0, l1, n1, l2, n2);
+}
+
+int orc_fir2_f32(const float* lobeH, int nH, const float* lobeV, int nV,
+                 const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+                 void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    return fir2<float>(lobeH, nH, lobeV, nV, in, in_dt, in_stride, nrow, ncol,
+                       out, out_dt, out_stride, shift, nthreads);
+}
+
+int orc_fir2_f64(const double* lobeH, int nH, const double* lobeV, int nV,
+                 const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+                 void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    return fir2<double>(lobeH, nH, lobeV, nV, in, in_dt, in_stride, nrow, ncol,
+                        out, out_dt, out_stride, shift, nthreads);
+}
+
+// Element conversions, exposed so the tests can pin them against numpy.
+float    orc_half_to_float(uint16_t h)   { return half_to_float(h); }
+uint16_t orc_float_to_half(float f)      { return float_to_half(f); }
+float    orc_bf16_to_float(uint16_t b)   { return bf16_to_float(b); }
+uint16_t orc_float_to_bf16(float f)      { return float_to_bf16(f); }
+uint8_t  orc_float_to_u8(float f)        { return to_u8(f); }
+
+}   // extern "C"

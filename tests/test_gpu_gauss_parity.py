@@ -1,0 +1,120 @@
+"""FIRGaussianConvolver2 parity on the GPU: the six methods one at a time
+(FIRGaussianConvolver.h:178-242) and fused into one launch, against the
+oracle's faithful FP32 restatement; at the full 640x480 and 1920x1080
+workloads as well as small ones."""
+import numpy as np
+import pytest
+
+import oracle
+from tests._util import assert_f32_parity, assert_u8_parity, sentinel, to_dev, to_host
+from tucuda_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SIGMAS = (0.8, 1.0, 1.2, 1.5, 2.0, 2.5, 3.0, 4.0)
+
+
+def _oracle_all(sigma, img, out_dtype=np.float32, shift=True):
+    lobes = oracle.gauss_lobes(sigma)
+    refs = {}
+    for m in oracle.METHOD_ORDER:
+        ref = np.full(img.shape, sentinel(out_dtype), dtype=out_dtype)
+        oracle.gauss(m, sigma, img, ref, shift, lobes=lobes, nthreads=8)
+        refs[m] = ref
+    return refs
+
+
+def test_lobes_match_oracle():
+    """The library's host lobes equal the oracle's bit for bit."""
+    import tucuda_amd
+    for sigma in np.linspace(0.5, 5.0, 91):
+        ours = tucuda_amd.gauss_lobes(float(sigma))
+        ref = oracle.gauss_lobes(float(np.float32(sigma)))
+        for a, b in zip(ours, ref):
+            assert a.size == b.size and (a.view(np.uint32) == b.view(np.uint32)).all(), sigma
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_methods_one_by_one(dev, sigma):
+    import tucuda_amd
+    img = synth.edge_image(11, 97, 301)
+    refs = _oracle_all(sigma, img)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    d_in = to_dev(img, dev)
+    for m in oracle.METHOD_ORDER:
+        d_out = to_dev(np.full(img.shape, sentinel(np.float32), np.float32), dev)
+        getattr(g, m)(d_in, d_out)
+        assert_f32_parity(to_host(d_out), refs[m], f"sigma={sigma} {m}")
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+@pytest.mark.parametrize("shift", [True, False])
+def test_fused_six(dev, sigma, shift):
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(12, 131, 517)
+    refs = _oracle_all(sigma, img, shift=shift)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    outs = {m: torch.full(img.shape, sentinel(np.float32), device=dev) for m in oracle.METHOD_ORDER}
+    g.convolve6(to_dev(img, dev), outs, shift)
+    for m in oracle.METHOD_ORDER:
+        assert_f32_parity(to_host(outs[m]), refs[m], f"fused sigma={sigma} shift={shift} {m}")
+
+
+@pytest.mark.parametrize("mask", [0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x05, 0x3A, 0x24])
+def test_fused_subsets(dev, mask):
+    """Any subset of outputs; outputs not asked for are not touched."""
+    import torch
+    import tucuda_amd
+    sigma = 2.5                           # lobe sizes (9, 16, 17): per-output regions differ
+    img = synth.edge_image(13, 90, 333)
+    refs = _oracle_all(sigma, img)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    outs = {m: torch.full(img.shape, sentinel(np.float32), device=dev) for m in oracle.METHOD_ORDER}
+    want = {m: outs[m] for i, m in enumerate(oracle.METHOD_ORDER) if mask >> i & 1}
+    g.convolve6(to_dev(img, dev), want)
+    for i, m in enumerate(oracle.METHOD_ORDER):
+        got = to_host(outs[m])
+        if mask >> i & 1:
+            assert_f32_parity(got, refs[m], f"mask={mask:#x} {m}")
+        else:
+            assert (got == sentinel(np.float32)).all()
+
+
+def test_u8_in_u8_out(dev):
+    """gaussTest's instantiation: u_char in, float intermediate, u_char out
+    (Testprog/gaussTest/main.cu:62-64)."""
+    import tucuda_amd
+    img = synth.u8_image(14, 120, 260)
+    refs_u8 = _oracle_all(2.0, img, np.uint8)
+    refs_f = _oracle_all(2.0, img, np.float32)
+    g = tucuda_amd.FIRGaussianConvolver2(2.0)
+    d_in = to_dev(img, dev)
+    for m in ("smooth", "diffH", "diffVV"):
+        d_out = to_dev(np.full(img.shape, sentinel(np.uint8), np.uint8), dev)
+        getattr(g, m)(d_in, d_out)
+        assert_u8_parity(to_host(d_out), refs_u8[m], refs_f[m], m)
+
+
+def test_workload_640x480_smooth(dev):
+    """640x480 float32, uniform [0,255], seed 0, sigma 2 smooth."""
+    import tucuda_amd
+    img = synth.uniform_image(0, 480, 640)
+    ref = np.full(img.shape, sentinel(np.float32), np.float32)
+    oracle.gauss("smooth", 2.0, img, ref, nthreads=8)
+    d_out = to_dev(np.full(img.shape, sentinel(np.float32), np.float32), dev)
+    tucuda_amd.FIRGaussianConvolver2(2.0).smooth(to_dev(img, dev), d_out)
+    assert_f32_parity(to_host(d_out), ref, "640x480 smooth")
+
+
+def test_workload_1080p_six(dev):
+    """1920x1080 float32 edge image, seed 1, sigma 3, all six outputs in one
+    call against the oracle."""
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(1, 1080, 1920)
+    refs = _oracle_all(3.0, img)
+    outs = {m: torch.full(img.shape, sentinel(np.float32), device=dev) for m in oracle.METHOD_ORDER}
+    tucuda_amd.FIRGaussianConvolver2(3.0).convolve6(to_dev(img, dev), outs)
+    for m in oracle.METHOD_ORDER:
+        assert_f32_parity(to_host(outs[m]), refs[m], f"1080p {m}")

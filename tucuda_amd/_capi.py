@@ -1,0 +1,119 @@
+"""ctypes binding of libtucuda_amd.so (the C ABI declared in include/tucu.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the shared object is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtucuda_amd.so")
+
+TUCU_OK = 0
+TUCU_ERR_INVALID = 1
+TUCU_ERR_LOBE_TOO_LARGE = 2
+TUCU_ERR_UNSUPPORTED_LOBE = 3
+TUCU_ERR_BAD_ALLOC = 4
+TUCU_ERR_DEVICE = 5
+
+TUCU_U8, TUCU_F32, TUCU_F16, TUCU_BF16 = 0, 1, 2, 3
+TUCU_RULE_REFERENCE, TUCU_RULE_CEIL4SIGMA = 0, 1
+TUCU_ALL6 = 0x3F
+
+# Every symbol include/tucu.h declares.
+EXPORTS = (
+    "tucu_last_error", "tucu_version", "tucu_malloc", "tucu_free", "tucu_row_stride",
+    "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
+    "tucu_lobe_size", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
+)
+
+
+class TucuImage(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("stride", ctypes.c_int64),
+        ("nrow", ctypes.c_int32),
+        ("ncol", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class TucuError(RuntimeError):
+    """A failure reported by the C ABI; ``code`` is the tucu_status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class LobeTooLarge(TucuError):
+    """FIRFilter.h:250-251 (runtime_error at initialize)."""
+
+
+class UnsupportedLobe(TucuError):
+    """FIRFilter.h:343-344, 374-375 (runtime_error at convolve)."""
+
+
+class BadAlloc(TucuError, MemoryError):
+    """allocator.h:83-84 (std::bad_alloc)."""
+
+
+class DeviceError(TucuError):
+    """A HIP runtime failure."""
+
+
+_ERRORS = {
+    TUCU_ERR_LOBE_TOO_LARGE: LobeTooLarge,
+    TUCU_ERR_UNSUPPORTED_LOBE: UnsupportedLobe,
+    TUCU_ERR_BAD_ALLOC: BadAlloc,
+    TUCU_ERR_DEVICE: DeviceError,
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or make -C tucuda_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    fp = ctypes.POINTER(ctypes.c_float)
+    ip = ctypes.POINTER(ctypes.c_int)
+    imgp = ctypes.POINTER(TucuImage)
+    L.tucu_last_error.restype = ctypes.c_char_p
+    L.tucu_last_error.argtypes = []
+    L.tucu_version.restype = ctypes.c_int
+    L.tucu_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.tucu_free.argtypes = [ctypes.c_void_p]
+    L.tucu_row_stride.restype = ctypes.c_size_t
+    L.tucu_row_stride.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+    L.tucu_memcpy2d.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+    L.tucu_synchronize.argtypes = []
+    L.tucu_fir2_check.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.tucu_fir2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, imgp, imgp,
+                                     ctypes.c_int, ctypes.c_void_p]
+    L.tucu_lobe_size.argtypes = [fp, ctypes.c_int]
+    L.tucu_gauss2_lobes.argtypes = [ctypes.c_float, ctypes.c_int, fp, ip, fp, ip, fp, ip]
+    L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,
+                                       imgp, imgp, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]
+    for name in EXPORTS:
+        if name != "tucu_last_error" and name != "tucu_row_stride" and name != "tucu_version":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    """Raise the exception matching a tucu_status."""
+    if rc == TUCU_OK:
+        return
+    msg = load().tucu_last_error().decode(errors="replace")
+    raise _ERRORS.get(rc, TucuError)(rc, msg)

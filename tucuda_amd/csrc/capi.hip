@@ -1,0 +1,317 @@
+// capi.hip -- the C ABI of libtucuda_amd.so (declared in include/tucu.h).
+//
+// Validation and error behaviour follow FIRFilter2 / FIRGaussianConvolver2:
+// lobe-length limit at initialize (FIRFilter.h:250-251), silent no-op for
+// images too small for the kernel (FIRFilter.h:304-312), unsupported lobe
+// length at convolve (FIRFilter.h:343-344, 374-375).  Where the reference
+// prints and exits on a device error (algorithm.h:85-94) this layer returns
+// TUCU_ERR_DEVICE and leaves the decision to the caller.
+#include "tucu.h"
+#include "sepfir.h"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace tucu {
+
+KernelEntry g_fir_table[4][4][4][4];
+KernelEntry g_gauss_table[5][4][4];
+
+size_t lobe_size(const float lobe[], bool even);
+void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
+void gauss_lobes_ceil4(float, float*, int*, float*, int*, float*, int*);
+
+namespace {
+
+thread_local std::string t_error;
+
+int fail(int code, const std::string& msg)
+{
+    t_error = msg;
+    return code;
+}
+
+int device_fail(hipError_t e, const char* what)
+{
+    return fail(TUCU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t dtype_size(int dt)
+{
+    switch (dt) {
+      case TUCU_U8:   return 1;
+      case TUCU_F32:  return 4;
+      case TUCU_F16:
+      case TUCU_BF16: return 2;
+      default:        return 0;
+    }
+}
+
+bool supported_lobe(int n)
+{
+    return n == 17 || n == 16 || n == 9 || n == 8 || n == 5 || n == 4 || n == 3 || n == 2;
+}
+
+int fir_class(int r)
+{
+    return r == 2 ? 0 : r == 4 ? 1 : r == 8 ? 2 : r == 16 ? 3 : -1;
+}
+
+int gauss_class(int r)
+{
+    for (int i = 0; i < 5; ++i)
+        if (r <= GAUSS_RADII[i]) return i;
+    return -1;
+}
+
+int check_image(const tucu_image* im, const char* name, bool need_data)
+{
+    if (!im) return fail(TUCU_ERR_INVALID, std::string(name) + " descriptor is null");
+    if (dtype_size(im->dtype) == 0) return fail(TUCU_ERR_INVALID, std::string(name) + ": unknown dtype");
+    if (im->nrow < 0 || im->ncol < 0) return fail(TUCU_ERR_INVALID, std::string(name) + ": negative size");
+    if (need_data && !im->data) return fail(TUCU_ERR_INVALID, std::string(name) + ": null data");
+    if (im->stride < im->ncol) return fail(TUCU_ERR_INVALID, std::string(name) + ": stride shorter than a row");
+    return TUCU_OK;
+}
+
+// A lobe in the reference layout (lobe[0] outermost, centre last for
+// symmetric lobes) turned into centre-indexed coefficients c[0..16]:
+// c[k] weights the taps at distance k, c[0] the centre (0 if antisymmetric).
+void centre_index(const float* lobe, int r, bool symmetric, float c[KMAX])
+{
+    std::fill(c, c + KMAX, 0.f);
+    for (int k = 1; k <= r; ++k) c[k] = lobe[r - k];
+    if (symmetric) c[0] = lobe[r];
+}
+
+void set_out(OutDesc& o, const tucu_image& im, int rv, int rh, int nrow, int ncol, bool shift)
+{
+    o.ptr = im.data;
+    o.stride = im.stride;
+    o.dy = shift ? 0 : -rv;
+    o.dx = shift ? 0 : -rh;
+    o.y0 = rv; o.y1 = nrow - rv;
+    o.x0 = rh; o.x1 = ncol - rh;
+}
+
+// The written rectangle must fit the output when its extent is known.
+int check_extent(const tucu_image& o, int rv, int rh, int nrow, int ncol, bool shift, const char* name)
+{
+    if (o.nrow == 0 && o.ncol == 0) return TUCU_OK;          // extent unknown, as in the reference
+    const int64_t rows = int64_t(shift ? rv : 0) + nrow - 2 * rv;
+    const int64_t cols = int64_t(shift ? rh : 0) + ncol - 2 * rh;
+    if (rows > o.nrow || cols > o.ncol)
+        return fail(TUCU_ERR_INVALID, std::string(name) + ": output smaller than the region written");
+    return TUCU_OK;
+}
+
+int launch(const KernelEntry& k, SepArgs& a, hipStream_t s)
+{
+    if (!k.fn) return fail(TUCU_ERR_INVALID, "no kernel for this type / radius combination");
+    const int two = NT - 2 * k.rh;
+    const int tiles_x = (a.cx1 - a.cx0 + two - 1) / two;
+    const int tiles_y = (a.cy1 - a.cy0 + k.th - 1) / k.th;
+    a.tiles_y = tiles_y;
+    const int64_t nblocks = int64_t(tiles_x) * tiles_y;
+    if (nblocks <= 0) return TUCU_OK;
+    if (nblocks > (int64_t(1) << 31) - 1) return fail(TUCU_ERR_INVALID, "image too large");
+    const hipError_t e = k.fn(a, int(nblocks), s);
+    if (e != hipSuccess) return device_fail(e, "kernel launch");
+    return TUCU_OK;
+}
+
+}   // namespace
+}   // namespace tucu
+
+using namespace tucu;
+
+extern "C" {
+
+const char* tucu_last_error(void) { return t_error.c_str(); }
+
+int tucu_version(void) { return 100; }
+
+int tucu_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(TUCU_ERR_INVALID, "null out pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return TUCU_OK;                 // allocator.h:74-75
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return fail(TUCU_ERR_BAD_ALLOC, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    const hipError_t e = hipMemset(p, 0, bytes);    // allocator.h:85
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return device_fail(e, "hipMemset");
+    }
+    *ptr = p;
+    return TUCU_OK;
+}
+
+int tucu_free(void* ptr)
+{
+    if (!ptr) return TUCU_OK;
+    const hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "hipFree");
+}
+
+size_t tucu_row_stride(size_t ncol, size_t elem_size)
+{
+    if (elem_size == 0) return ncol;
+    const size_t bytes = (ncol * elem_size + 255) / 256 * 256;
+    return bytes / elem_size;
+}
+
+int tucu_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch,
+                  size_t width_bytes, size_t height, int kind)
+{
+    if (width_bytes == 0 || height == 0) return TUCU_OK;
+    if (!dst || !src) return fail(TUCU_ERR_INVALID, "null pointer");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    const hipError_t e = hipMemcpy2D(dst, dpitch, src, spitch, width_bytes, height, k);
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "hipMemcpy2D");
+}
+
+int tucu_synchronize(void)
+{
+    const hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "hipDeviceSynchronize");
+}
+
+int tucu_fir2_check(int nH, int nV)
+{
+    if (nH < 0 || nV < 0) return fail(TUCU_ERR_INVALID, "negative lobe size");
+    if (nH > 17 || nV > 17)
+        return fail(TUCU_ERR_LOBE_TOO_LARGE, "FIRFilter2<T, BLOCK_TRAITS>::initialize: too large lobe size!");
+    return TUCU_OK;
+}
+
+int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
+                       const tucu_image* in, const tucu_image* out, int shift, void* stream)
+{
+    int rc = tucu_fir2_check(nH, nV);
+    if (rc) return rc;
+    if ((rc = check_image(in, "input", false))) return rc;
+    if ((rc = check_image(out, "output", false))) return rc;
+    const int nrow = in->nrow, ncol = in->ncol;
+    const int rV = nV & ~1, rH = nH & ~1;
+    if (nrow <= 2 * rV || ncol <= 2 * rH) return TUCU_OK;           // FIRFilter.h:304-312
+    if (!supported_lobe(nH))
+        return fail(TUCU_ERR_UNSUPPORTED_LOBE,
+                    "FIRFilter2::convolve: unsupported horizontal lobe size[" + std::to_string(nH) + "]!");
+    if (!supported_lobe(nV))
+        return fail(TUCU_ERR_UNSUPPORTED_LOBE,
+                    "FIRFilter2::convolve: unsupported vertical lobe size[" + std::to_string(nV) + "!");
+    if (!lobeH || !lobeV) return fail(TUCU_ERR_INVALID, "null lobe");
+    if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
+    if ((rc = check_extent(*out, rV, rH, nrow, ncol, shift != 0, "output"))) return rc;
+
+    SepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data;
+    a.in_stride = in->stride;
+    a.nrow = nrow; a.ncol = ncol;
+    a.cy0 = rV; a.cy1 = nrow - rV;
+    a.cx0 = rH; a.cx1 = ncol - rH;
+    a.mask = 1;
+    a.sgnv = (nV & 1) ? 1.f : -1.f;
+    a.sgnh = (nH & 1) ? 1.f : -1.f;
+    centre_index(lobeV, rV, nV & 1, a.cv[0]);
+    centre_index(lobeH, rH, nH & 1, a.ch[0]);
+    set_out(a.out[0], *out, rV, rH, nrow, ncol, shift != 0);
+    const KernelEntry& k = g_fir_table[fir_class(rV)][fir_class(rH)][in->dtype][out->dtype];
+    return launch(k, a, static_cast<hipStream_t>(stream));
+}
+
+int tucu_lobe_size(const float* lobe17, int even)
+{
+    if (!lobe17) return fail(TUCU_ERR_INVALID, "null lobe");
+    return int(lobe_size(lobe17, even != 0));
+}
+
+int tucu_gauss2_lobes(float sigma, int rule, float* lobe0, int* n0, float* lobe1, int* n1,
+                      float* lobe2, int* n2)
+{
+    if (!lobe0 || !lobe1 || !lobe2 || !n0 || !n1 || !n2) return fail(TUCU_ERR_INVALID, "null pointer");
+    if (rule == TUCU_RULE_REFERENCE) {
+        gauss_lobes_reference(sigma, lobe0, n0, lobe1, n1, lobe2, n2);
+        return TUCU_OK;
+    }
+    if (rule == TUCU_RULE_CEIL4SIGMA) {
+        if (!(sigma > 0.f) || !(sigma < 1e6f)) return fail(TUCU_ERR_INVALID, "sigma must be positive and finite");
+        gauss_lobes_ceil4(sigma, lobe0, n0, lobe1, n1, lobe2, n2);
+        return TUCU_OK;
+    }
+    return fail(TUCU_ERR_INVALID, "unknown lobe rule");
+}
+
+int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
+                         const float* lobe2, int n2,
+                         const tucu_image* in, const tucu_image* outs,
+                         unsigned mask, int shift, void* stream)
+{
+    int rc;
+    if ((rc = check_image(in, "input", false))) return rc;
+    if (mask & ~unsigned(TUCU_ALL6)) return fail(TUCU_ERR_INVALID, "unknown output bits in mask");
+    if (mask == 0) return TUCU_OK;
+    if (!outs) return fail(TUCU_ERR_INVALID, "null output array");
+    const int r[3] = {n0 - 1, n1, n2 - 1};
+    const float* lobes[3] = {lobe0, lobe1, lobe2};
+    static const int HV[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};   // (H lobe, V lobe)
+    const int nrow = in->nrow, ncol = in->ncol;
+
+    unsigned live = 0;
+    int rmax = 0, cy0 = nrow, cy1 = 0, cx0 = ncol, cx1 = 0;
+    for (int k = 0; k < 6; ++k) {
+        if (!(mask & (1u << k))) continue;
+        const int h = HV[k][0], v = HV[k][1];
+        for (int l : {h, v}) {
+            if (r[l] < 1 || r[l] > 16)
+                return fail(TUCU_ERR_UNSUPPORTED_LOBE, "lobe " + std::to_string(l) + " length outside the supported range");
+            if (!lobes[l]) return fail(TUCU_ERR_INVALID, "null lobe");
+        }
+        const int rv = r[v], rh = r[h];
+        if (nrow <= 2 * rv || ncol <= 2 * rh) continue;               // FIRFilter.h:304-312
+        const std::string name = "output " + std::to_string(k);
+        if ((rc = check_image(&outs[k], name.c_str(), true))) return rc;
+        if ((rc = check_extent(outs[k], rv, rh, nrow, ncol, shift != 0, name.c_str()))) return rc;
+        live |= 1u << k;
+        rmax = std::max(rmax, std::max(rv, rh));
+        cy0 = std::min(cy0, rv); cy1 = std::max(cy1, nrow - rv);
+        cx0 = std::min(cx0, rh); cx1 = std::max(cx1, ncol - rh);
+    }
+    if (!live) return TUCU_OK;
+    if (!in->data) return fail(TUCU_ERR_INVALID, "input: null data");
+
+    SepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data;
+    a.in_stride = in->stride;
+    a.nrow = nrow; a.ncol = ncol;
+    a.cy0 = cy0; a.cy1 = cy1; a.cx0 = cx0; a.cx1 = cx1;
+    a.mask = live;
+    a.sgnv = a.sgnh = 1.f;
+    for (int l = 0; l < 3; ++l) {
+        if (r[l] < 1 || r[l] > 16 || !lobes[l]) continue;
+        centre_index(lobes[l], r[l], l != 1, a.cv[l]);
+        centre_index(lobes[l], r[l], l != 1, a.ch[l]);
+    }
+    for (int k = 0; k < 6; ++k)
+        if (live & (1u << k)) set_out(a.out[k], outs[k], r[HV[k][1]], r[HV[k][0]], nrow, ncol, shift != 0);
+    int odt = -1;
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        if (odt < 0) odt = outs[k].dtype;
+        else if (outs[k].dtype != odt)
+            return fail(TUCU_ERR_INVALID, "all outputs of one call must share an element type");
+    }
+    const KernelEntry& e = g_gauss_table[gauss_class(rmax)][in->dtype][odt];
+    return launch(e, a, static_cast<hipStream_t>(stream));
+}
+
+}   // extern "C"

@@ -1,0 +1,4 @@
+// FIRFilter2 kernels reading u8 images.
+#define IN_T uint8_t
+#define IN_IDX 0
+#include "inst_fir.inc"

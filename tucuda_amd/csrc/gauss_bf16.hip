@@ -1,0 +1,4 @@
+// Fused Gaussian kernels reading bf16 images.
+#define IN_T bf16_t
+#define IN_IDX 3
+#include "inst_gauss.inc"

@@ -1,0 +1,207 @@
+"""Python mirror of TU::cu::FIRFilter2 and TU::cu::FIRGaussianConvolver2.
+
+Same method names, argument meanings and error behaviour as the C++
+classes (TU/cu/FIRFilter.h:18-52, TU/cu/FIRGaussianConvolver.h:56-90);
+images are 2-D device tensors (rows contiguous, any row stride) standing in
+for ``cu::Array2<T>`` row ranges.  All work is enqueued on torch's current
+HIP stream through the C ABI; nothing here computes on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Mapping, Optional, Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import TucuImage, check
+
+_DTYPES = {
+    torch.uint8: _capi.TUCU_U8,
+    torch.float32: _capi.TUCU_F32,
+    torch.float16: _capi.TUCU_F16,
+    torch.bfloat16: _capi.TUCU_BF16,
+}
+
+METHOD_ORDER = ("smooth", "diffH", "diffV", "diffHH", "diffHV", "diffVV")
+METHOD_BITS = {m: 1 << i for i, m in enumerate(METHOD_ORDER)}
+# (horizontal lobe, vertical lobe): FIRGaussianConvolver.h:181, 193, 205, 217, 229, 241
+METHOD_LOBES = {"smooth": (0, 0), "diffH": (1, 0), "diffV": (0, 1),
+                "diffHH": (2, 0), "diffHV": (1, 1), "diffVV": (0, 2)}
+
+
+def image_desc(t: torch.Tensor, with_extent: bool = True) -> TucuImage:
+    """The C descriptor of a 2-D device tensor."""
+    if not isinstance(t, torch.Tensor) or t.dim() != 2:
+        raise ValueError("images are 2-D tensors")
+    if t.dtype not in _DTYPES:
+        raise TypeError(f"unsupported element type {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError("images must live in device memory")
+    if t.shape[1] > 1 and t.stride(1) != 1:
+        raise ValueError("image rows must be contiguous")
+    stride = t.stride(0) if t.shape[0] > 1 else max(t.shape[1], 1)
+    return TucuImage(t.data_ptr(), stride,
+                     t.shape[0] if with_extent else 0,
+                     t.shape[1] if with_extent else 0,
+                     _DTYPES[t.dtype], 0)
+
+
+def _stream_handle(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _f32(lobe) -> np.ndarray:
+    if isinstance(lobe, torch.Tensor):
+        lobe = lobe.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(lobe, dtype=np.float32).reshape(-1))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class FIRFilter2:
+    """Separable 2-D FIR filter with symmetric (odd length) or antisymmetric
+    (even length) half-kernels -- TU::cu::FIRFilter2 (FIRFilter.h:18-52)."""
+
+    LobeSizeMax = 17                          # FIRFilter.h:26
+
+    def __init__(self):
+        self._lobeH = np.zeros(0, np.float32)
+        self._lobeV = np.zeros(0, np.float32)
+
+    def initialize(self, lobeH, lobeV) -> "FIRFilter2":
+        """FIRFilter.h:245-261: LobeTooLarge (a RuntimeError) when a lobe is
+        longer than 17; the lengths are validated against the supported set
+        at convolve time."""
+        lh, lv = _f32(lobeH), _f32(lobeV)
+        check(_capi.load().tucu_fir2_check(lh.size, lv.size))
+        self._lobeH, self._lobeV = lh, lv
+        return self
+
+    def convolve(self, inp: torch.Tensor, out: torch.Tensor, shift: bool = True) -> None:
+        """FIRFilter.h:298-377: writes the (nrow-2*offsetV) x (ncol-2*offsetH)
+        valid region of ``out`` (at (offsetV, offsetH) if ``shift``), leaves
+        the rest untouched; images too small for the kernel are a no-op."""
+        src, dst = image_desc(inp), image_desc(out)
+        if inp.device != out.device:
+            raise ValueError("input and output must be on the same device")
+        check(_capi.load().tucu_fir2_convolve(
+            _fptr(self._lobeH), self._lobeH.size, _fptr(self._lobeV), self._lobeV.size,
+            ctypes.byref(src), ctypes.byref(dst), int(bool(shift)), _stream_handle(inp)))
+
+    def outSizeH(self, inSize: int) -> int:     # FIRFilter.h:37
+        return inSize - 2 * self.offsetH()
+
+    def outSizeV(self, inSize: int) -> int:     # FIRFilter.h:38
+        return inSize - 2 * self.offsetV()
+
+    def offsetH(self) -> int:                   # FIRFilter.h:39
+        return self._lobeH.size & ~1
+
+    def offsetV(self) -> int:                   # FIRFilter.h:40
+        return self._lobeV.size & ~1
+
+
+def gauss_lobes(sigma: float, rule: int = _capi.TUCU_RULE_REFERENCE):
+    """FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)
+    (FIRGaussianConvolver.h:123-170), computed by the library on the host."""
+    bufs = [np.zeros(17, np.float32) for _ in range(3)]
+    ns = [ctypes.c_int(0) for _ in range(3)]
+    check(_capi.load().tucu_gauss2_lobes(
+        ctypes.c_float(sigma), rule,
+        _fptr(bufs[0]), ctypes.byref(ns[0]), _fptr(bufs[1]), ctypes.byref(ns[1]),
+        _fptr(bufs[2]), ctypes.byref(ns[2])))
+    return tuple(b[: n.value].copy() for b, n in zip(bufs, ns))
+
+
+def lobe_size(lobe17, even: bool) -> int:
+    """detail::lobeSize (FIRGaussianConvolver.cu:52-98)."""
+    a = _f32(lobe17)
+    if a.size != 17:
+        raise ValueError("lobeSize takes 17 samples")
+    return _capi.load().tucu_lobe_size(_fptr(a), int(bool(even)))
+
+
+OutputSpec = Union[Mapping[str, torch.Tensor], Sequence[Optional[torch.Tensor]]]
+
+
+class FIRGaussianConvolver2(FIRFilter2):
+    """Gaussian smoothing and derivatives -- TU::cu::FIRGaussianConvolver2
+    (FIRGaussianConvolver.h:56-90)."""
+
+    def __init__(self, sigma: float = 1.0, rule: int = _capi.TUCU_RULE_REFERENCE):
+        super().__init__()
+        self._rule = rule
+        self.initialize(sigma)
+
+    def sigma(self) -> float:
+        return self._sigma
+
+    def initialize(self, sigma: float) -> "FIRGaussianConvolver2":      # :114-121
+        self._sigma = float(np.float32(sigma))
+        self._lobe0, self._lobe1, self._lobe2 = gauss_lobes(self._sigma, self._rule)
+        return self
+
+    @staticmethod
+    def lobes(sigma: float, rule: int = _capi.TUCU_RULE_REFERENCE):
+        """The static initialize(sigma, lobe0, lobe1, lobe2) (:123-170)."""
+        return gauss_lobes(sigma, rule)
+
+    def _lobes(self):
+        return (self._lobe0, self._lobe1, self._lobe2)
+
+    def _method(self, name: str, inp, out, shift):
+        if self._rule != _capi.TUCU_RULE_REFERENCE:
+            self.convolve6(inp, {name: out}, shift)
+            return
+        h, v = METHOD_LOBES[name]
+        lobes = self._lobes()
+        FIRFilter2.initialize(self, lobes[h], lobes[v]).convolve(inp, out, shift)
+
+    def smooth(self, inp, out, shift: bool = True):   # :178-182
+        self._method("smooth", inp, out, shift)
+
+    def diffH(self, inp, out, shift: bool = True):    # :190-194
+        self._method("diffH", inp, out, shift)
+
+    def diffV(self, inp, out, shift: bool = True):    # :202-206
+        self._method("diffV", inp, out, shift)
+
+    def diffHH(self, inp, out, shift: bool = True):   # :214-218
+        self._method("diffHH", inp, out, shift)
+
+    def diffHV(self, inp, out, shift: bool = True):   # :226-230
+        self._method("diffHV", inp, out, shift)
+
+    def diffVV(self, inp, out, shift: bool = True):   # :238-242
+        self._method("diffVV", inp, out, shift)
+
+    def convolve6(self, inp: torch.Tensor, outs: OutputSpec, shift: bool = True) -> None:
+        """Extension: every requested method from one fused launch.  ``outs``
+        maps method names to output tensors, or lists up to six tensors in
+        method order (None = not wanted).  Each output gets exactly what the
+        corresponding single method would write."""
+        if isinstance(outs, Mapping):
+            seq = [outs.get(m) for m in METHOD_ORDER]
+            unknown = set(outs) - set(METHOD_ORDER)
+            if unknown:
+                raise KeyError(f"unknown methods {sorted(unknown)}")
+        else:
+            seq = list(outs) + [None] * (6 - len(outs))
+        descs = (TucuImage * 6)()
+        mask = 0
+        for k, t in enumerate(seq[:6]):
+            if t is None:
+                continue
+            if t.device != inp.device:
+                raise ValueError("input and outputs must be on the same device")
+            descs[k] = image_desc(t)
+            mask |= 1 << k
+        src = image_desc(inp)
+        l0, l1, l2 = self._lobes()
+        check(_capi.load().tucu_gauss2_convolve(
+            _fptr(l0), l0.size, _fptr(l1), l1.size, _fptr(l2), l2.size,
+            ctypes.byref(src), descs, mask, int(bool(shift)), _stream_handle(inp)))
