@@ -75,6 +75,10 @@ def main():
     ap.add_argument("--sets", type=int, default=10, help="rotating buffer sets (>= 5 defeats the 256 MB MALL)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--nrow", type=int, default=NROW, help="image rows (the headline workload is 1080)")
+    ap.add_argument("--ncol", type=int, default=NCOL, help="image columns (the headline workload is 1920)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -87,22 +91,30 @@ def main():
     from tucuda_amd import synth
 
     dev = torch.device("cuda:0")
-    img = synth.edge_image(1, NROW, NCOL)
+    nrow, ncol = args.nrow, args.ncol
+    img = synth.edge_image(1, nrow, ncol)
     g = tucuda_amd.FIRGaussianConvolver2(SIGMA)
     l0, l1, l2 = (x.size for x in tucuda_amd.gauss_lobes(SIGMA))
     r = [l0 - 1, l1, l2 - 1]
     radii = [(r[v], r[h]) for h, v in (tucuda_amd.METHOD_LOBES[m] for m in tucuda_amd.METHOD_ORDER)]
-    alg_bytes = algorithmic_bytes(NROW, NCOL, radii)
+    alg_bytes = algorithmic_bytes(nrow, ncol, radii)
 
     d_img = torch.from_numpy(img).to(dev)
     sets = []
     for s in range(args.sets):
         ins = d_img.clone()
-        outs = [torch.empty((NROW, NCOL), dtype=torch.float32, device=dev) for _ in range(6)]
+        outs = [torch.empty((nrow, ncol), dtype=torch.float32, device=dev) for _ in range(6)]
         sets.append((ins, outs))
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream(dev)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for k in range(args.warmup):
         ins, outs = sets[k % args.sets]
         g.convolve6(ins, outs)
@@ -126,7 +138,7 @@ def main():
     avg_kern_ms = float(np.mean(kern_ms))
     med_kern_ms = float(np.median(kern_ms))
     ms_per_step = total_ms / args.steps
-    gpix = NROW * NCOL / (ms_per_step * 1e-3) / 1e9
+    gpix = nrow * ncol / (ms_per_step * 1e-3) / 1e9
     achieved = alg_bytes / (avg_kern_ms * 1e-3) / 1e9
     log(f"kernel avg {avg_kern_ms*1e3:.2f} us median {med_kern_ms*1e3:.2f} us; "
         f"wall per step {ms_per_step*1e3:.2f} us; {achieved:.0f} GB/s algorithmic")
@@ -151,9 +163,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded uniform [0,255] + 8 step edges)",
         "config": {
-            "workload": "1920x1080 float32 image, sigma=3 (lobes 17/16/17, radius 16), "
+            "workload": f"{ncol}x{nrow} float32 image, sigma=3 (lobes 17/16/17, radius 16), "
                         "smooth+diffH+diffV+diffHH+diffHV+diffVV fused in one launch",
-            "nrow": NROW, "ncol": NCOL, "sigma": SIGMA, "outputs": 6,
+            "nrow": nrow, "ncol": ncol, "sigma": SIGMA, "outputs": 6,
             "buffer_sets": args.sets, "shift": True,
         },
         "roofline": {

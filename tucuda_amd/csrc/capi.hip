@@ -11,13 +11,15 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
 namespace tucu {
 
 KernelEntry g_fir_table[4][4][4][4];
-KernelEntry g_gauss_table[5][4][4];
+KernelEntry g_gauss_table[5][NGTH][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
@@ -66,6 +68,20 @@ int gauss_class(int r)
     return -1;
 }
 
+// Tile-height choice of the fused Gaussian kernel: TUCU_GAUSS_TH (one of
+// GAUSS_THS) overrides the default.
+int gauss_th_index()
+{
+    static const int idx = [] {
+        const char* e = std::getenv("TUCU_GAUSS_TH");
+        if (e)
+            for (int i = 0; i < NGTH; ++i)
+                if (std::atoi(e) == GAUSS_THS[i]) return i;
+        return GAUSS_TH_DEFAULT;
+    }();
+    return idx;
+}
+
 int check_image(const tucu_image* im, const char* name, bool need_data)
 {
     if (!im) return fail(TUCU_ERR_INVALID, std::string(name) + " descriptor is null");
@@ -86,14 +102,42 @@ void centre_index(const float* lobe, int r, bool symmetric, float c[KMAX])
     if (symmetric) c[0] = lobe[r];
 }
 
-void set_out(OutDesc& o, const tucu_image& im, int rv, int rh, int nrow, int ncol, bool shift)
+// Output k of the kernel: base address of centre (0, 0) and row stride.
+int set_out(SepArgs& a, int k, const tucu_image& im, int rv, int rh, bool shift)
 {
-    o.ptr = im.data;
-    o.stride = im.stride;
-    o.dy = shift ? 0 : -rv;
-    o.dx = shift ? 0 : -rh;
-    o.y0 = rv; o.y1 = nrow - rv;
-    o.x0 = rh; o.x1 = ncol - rh;
+    if (im.stride > INT32_MAX) return fail(TUCU_ERR_INVALID, "output row stride too large");
+    const int64_t es = int64_t(dtype_size(im.dtype));
+    const int64_t dy = shift ? 0 : -rv, dx = shift ? 0 : -rh;
+    a.obase[k] = reinterpret_cast<uintptr_t>(im.data) + uintptr_t((dy * im.stride + dx) * es);
+    a.ostride[k] = int32_t(im.stride);
+    return TUCU_OK;
+}
+
+// Input buffer and the fast-store rectangle of a launch whose live outputs
+// are `live`, output k valid on centre rows [rv[k], nrow - rv[k]) and
+// columns [rh[k], ncol - rh[k]).
+int finish_args(SepArgs& a, const tucu_image& in, unsigned live, const tucu_image* outs,
+                const int* rv, const int* rh)
+{
+    const int64_t es = int64_t(dtype_size(in.dtype));
+    const int64_t stride_b = in.stride * es;
+    if (int64_t(in.nrow) * stride_b > INT32_MAX)
+        return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+    a.in = in.data;
+    a.in_stride_b = int32_t(stride_b);
+    a.in_bytes = uint32_t(int64_t(in.nrow - 1) * stride_b + in.ncol * es);
+    a.nrow = in.nrow; a.ncol = in.ncol;
+    a.fy0 = a.fx0 = 0; a.fy1 = in.nrow; a.fx1 = in.ncol;
+    bool aligned = true;
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        a.fy0 = std::max(a.fy0, rv[k]); a.fy1 = std::min(a.fy1, in.nrow - rv[k]);
+        a.fx0 = std::max(a.fx0, rh[k]); a.fx1 = std::min(a.fx1, in.ncol - rh[k]);
+        const uintptr_t oes = dtype_size(outs[k].dtype);
+        if (a.ostride[k] % 4 != 0 || (a.obase[k] + uintptr_t(a.cx0) * oes) % (4 * oes) != 0) aligned = false;
+    }
+    if (!aligned) a.fy0 = a.fy1 = a.fx0 = a.fx1 = 0;
+    return TUCU_OK;
 }
 
 // The written rectangle must fit the output when its extent is known.
@@ -213,17 +257,16 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
 
     SepArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.in = in->data;
-    a.in_stride = in->stride;
-    a.nrow = nrow; a.ncol = ncol;
     a.cy0 = rV; a.cy1 = nrow - rV;
     a.cx0 = rH; a.cx1 = ncol - rH;
     a.mask = 1;
     a.sgnv = (nV & 1) ? 1.f : -1.f;
     a.sgnh = (nH & 1) ? 1.f : -1.f;
+    a.r[0] = rV; a.r[1] = rH;
     centre_index(lobeV, rV, nV & 1, a.cv[0]);
-    centre_index(lobeH, rH, nH & 1, a.ch[0]);
-    set_out(a.out[0], *out, rV, rH, nrow, ncol, shift != 0);
+    centre_index(lobeH, rH, nH & 1, a.ch);
+    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
+    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
     const KernelEntry& k = g_fir_table[fir_class(rV)][fir_class(rH)][in->dtype][out->dtype];
     return launch(k, a, static_cast<hipStream_t>(stream));
 }
@@ -290,19 +333,21 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
 
     SepArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.in = in->data;
-    a.in_stride = in->stride;
-    a.nrow = nrow; a.ncol = ncol;
     a.cy0 = cy0; a.cy1 = cy1; a.cx0 = cx0; a.cx1 = cx1;
     a.mask = live;
     a.sgnv = a.sgnh = 1.f;
+    int rvk[6] = {0}, rhk[6] = {0};
     for (int l = 0; l < 3; ++l) {
         if (r[l] < 1 || r[l] > 16 || !lobes[l]) continue;
+        a.r[l] = r[l];
         centre_index(lobes[l], r[l], l != 1, a.cv[l]);
-        centre_index(lobes[l], r[l], l != 1, a.ch[l]);
     }
-    for (int k = 0; k < 6; ++k)
-        if (live & (1u << k)) set_out(a.out[k], outs[k], r[HV[k][1]], r[HV[k][0]], nrow, ncol, shift != 0);
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        rvk[k] = r[HV[k][1]]; rhk[k] = r[HV[k][0]];
+        if ((rc = set_out(a, k, outs[k], rvk[k], rhk[k], shift != 0))) return rc;
+    }
+    if ((rc = finish_args(a, *in, live, outs, rvk, rhk))) return rc;
     int odt = -1;
     for (int k = 0; k < 6; ++k) {
         if (!(live & (1u << k))) continue;
@@ -310,8 +355,9 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
         else if (outs[k].dtype != odt)
             return fail(TUCU_ERR_INVALID, "all outputs of one call must share an element type");
     }
-    const KernelEntry& e = g_gauss_table[gauss_class(rmax)][in->dtype][odt];
-    return launch(e, a, static_cast<hipStream_t>(stream));
+    const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
+    if (!e->fn) e = &g_gauss_table[gauss_class(rmax)][GAUSS_TH_DEFAULT][in->dtype][odt];
+    return launch(*e, a, static_cast<hipStream_t>(stream));
 }
 
 }   // extern "C"
