@@ -1,0 +1,86 @@
+"""The C ABI without a GPU: the library loads, exports every entry point
+include/tucu.h declares, and its host-only functions (lobe generation, size
+rule, argument checks, row stride) behave as the reference's host code."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from tucuda_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "tucu.h")).read()
+    return sorted(set(re.findall(r"TUCU_API\s+[\w\s\*]+?\b(tucu_\w+)\s*\(", text)))
+
+
+def test_header_lists_every_export():
+    assert declared_entry_points() == sorted(_capi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.load()
+    for name in declared_entry_points():
+        assert hasattr(lib, name), name
+    out = os.popen(f"nm -D --defined-only {_capi.LIB_PATH}").read()
+    exported = set(re.findall(r"\bT (tucu_\w+)", out))
+    assert exported == set(declared_entry_points())
+
+
+def test_version_and_errors():
+    lib = _capi.load()
+    assert lib.tucu_version() >= 100
+    assert lib.tucu_fir2_check(17, 17) == 0
+    assert lib.tucu_fir2_check(18, 3) == _capi.TUCU_ERR_LOBE_TOO_LARGE
+    assert b"too large lobe size" in lib.tucu_last_error()
+    with pytest.raises(_capi.LobeTooLarge):
+        _capi.check(lib.tucu_fir2_check(3, 20))
+    assert lib.tucu_fir2_check(-1, 3) == _capi.TUCU_ERR_INVALID
+
+
+def test_row_stride_256_byte_rows():
+    lib = _capi.load()
+    for ncol, es in [(1, 1), (255, 1), (256, 1), (257, 1), (640, 4), (1920, 4), (100, 2), (65, 4)]:
+        s = lib.tucu_row_stride(ncol, es)
+        assert s >= ncol and (s * es) % 256 == 0 and (s - ncol) * es < 256
+
+
+def test_host_lobes_equal_oracle_bitwise():
+    """tucu_gauss2_lobes (reference rule) reproduces the oracle's float
+    lobes bit for bit over a dense sigma sweep."""
+    import tucuda_amd
+    for sigma in np.linspace(0.5, 5.0, 181):
+        s = float(np.float32(sigma))
+        ours = tucuda_amd.gauss_lobes(s)
+        ref = oracle.gauss_lobes(s)
+        for a, b in zip(ours, ref):
+            assert a.size == b.size and (a.view(np.uint32) == b.view(np.uint32)).all(), sigma
+
+
+def test_lobe_size_equals_oracle():
+    import tucuda_amd
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        lobe = (rng.standard_normal(17) * np.exp(-rng.uniform(0, 1) * np.arange(17))).astype(np.float32)
+        for even in (True, False):
+            assert tucuda_amd.lobe_size(lobe, even) == oracle.lobe_size(lobe, even)
+
+
+def test_ceil4sigma_rule():
+    """Extension rule: radius ceil(4 sigma) capped at 16, the reference's
+    normalisations (unit DC gain for lobe 0, quadratic response 2)."""
+    import tucuda_amd
+    for sigma in (0.8, 1.0, 1.3, 2.2, 3.0, 3.99, 4.0):
+        l0, l1, l2 = tucuda_amd.gauss_lobes(sigma, tucuda_amd.TUCU_RULE_CEIL4SIGMA)
+        r = min(16, int(np.ceil(4 * np.float32(sigma))))
+        assert (l0.size, l1.size, l2.size) == (r + 1, r, r + 1)
+        k0 = np.concatenate([l0[:-1], l0[-1:], l0[:-1][::-1]]).astype(np.float64)
+        assert abs(k0.sum() - 1) < 2e-6
+        dist = np.arange(r, 0, -1, dtype=np.float64)
+        assert abs(2 * np.dot(l2[:-1].astype(np.float64), dist ** 2) - 2.0) < 1e-4
+        assert (l1 < 0).all()
