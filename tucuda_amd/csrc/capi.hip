@@ -140,13 +140,13 @@ int finish_args(SepArgs& a, const tucu_image& in, unsigned live, const tucu_imag
     return TUCU_OK;
 }
 
-// The written rectangle must fit the output when its extent is known.
+// The written rectangle must fit the output in each dimension whose extent
+// is known (nonzero); the reference itself checks neither.
 int check_extent(const tucu_image& o, int rv, int rh, int nrow, int ncol, bool shift, const char* name)
 {
-    if (o.nrow == 0 && o.ncol == 0) return TUCU_OK;          // extent unknown, as in the reference
     const int64_t rows = int64_t(shift ? rv : 0) + nrow - 2 * rv;
     const int64_t cols = int64_t(shift ? rh : 0) + ncol - 2 * rh;
-    if (rows > o.nrow || cols > o.ncol)
+    if ((o.nrow > 0 && rows > o.nrow) || (o.ncol > 0 && cols > o.ncol))
         return fail(TUCU_ERR_INVALID, std::string(name) + ": output smaller than the region written");
     return TUCU_OK;
 }
