@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--sets", type=int, default=10, help="rotating buffer sets (>= 5 defeats the 256 MB MALL)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step from Python instead of replaying a captured HIP graph")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--nrow", type=int, default=NROW, help="image rows (the headline workload is 1080)")
     ap.add_argument("--ncol", type=int, default=NCOL, help="image columns (the headline workload is 1920)")
@@ -120,28 +122,66 @@ def main():
         g.convolve6(ins, outs)
     torch.cuda.synchronize()
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_start.record(stream)
-    for k in range(args.steps):
-        ins, outs = sets[k % args.sets]
-        starts[k].record(stream)
-        g.convolve6(ins, outs)
-        ends[k].record(stream)
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = t_start.elapsed_time(t_end)
-    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    # Timed region.  Default: one HIP graph holding one launch per buffer
+    # set, replayed until `steps` launches have run (graph replay removes the
+    # per-call host overhead of Python + ctypes).  Events on the launch stream
+    # bracket each replay; the kernel-average duration is replay time /
+    # launches per replay.
+    if args.eager:
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_start.record(stream)
+        for k in range(args.steps):
+            ins, outs = sets[k % args.sets]
+            starts[k].record(stream)
+            g.convolve6(ins, outs)
+            ends[k].record(stream)
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        total_ms = t_start.elapsed_time(t_end)
+        kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+        steps = args.steps
+    else:
+        per = args.sets
+        graph = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream(dev)
+        cap.wait_stream(stream)
+        with torch.cuda.stream(cap):
+            with torch.cuda.graph(graph, stream=cap):
+                for k in range(per):
+                    ins, outs = sets[k]
+                    g.convolve6(ins, outs)
+        stream.wait_stream(cap)
+        for _ in range(2):
+            graph.replay()
+        torch.cuda.synchronize()
+        reps = max(1, args.steps // per)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        rs = torch.cuda.current_stream(dev)
+        t_start.record(rs)
+        for r in range(reps):
+            starts[r].record(rs)
+            graph.replay()
+            ends[r].record(rs)
+        t_end.record(rs)
+        torch.cuda.synchronize()
+        total_ms = t_start.elapsed_time(t_end)
+        kern_ms = [s.elapsed_time(e) / per for s, e in zip(starts, ends)]
+        steps = reps * per
     avg_kern_ms = float(np.mean(kern_ms))
     med_kern_ms = float(np.median(kern_ms))
-    ms_per_step = total_ms / args.steps
+    ms_per_step = total_ms / steps
     gpix = nrow * ncol / (ms_per_step * 1e-3) / 1e9
     achieved = alg_bytes / (avg_kern_ms * 1e-3) / 1e9
     log(f"kernel avg {avg_kern_ms*1e3:.2f} us median {med_kern_ms*1e3:.2f} us; "
-        f"wall per step {ms_per_step*1e3:.2f} us; {achieved:.0f} GB/s algorithmic")
+        f"wall per step {ms_per_step*1e3:.2f} us; {achieved:.0f} GB/s algorithmic "
+        f"({'eager' if args.eager else 'graph'} launches)")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "gauss6_1080p_f32_traffic.json")
@@ -154,7 +194,7 @@ def main():
         "value": round(gpix, 4),
         "unit": "Gpix/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 6),
         "higher_is_better": True,
@@ -167,6 +207,7 @@ def main():
                         "smooth+diffH+diffV+diffHH+diffHV+diffVV fused in one launch",
             "nrow": nrow, "ncol": ncol, "sigma": SIGMA, "outputs": 6,
             "buffer_sets": args.sets, "shift": True,
+            "launch": "eager" if args.eager else "hip-graph replay",
         },
         "roofline": {
             "bound": "hbm",

@@ -121,6 +121,17 @@ TUCU_API int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, 
                                 const tucu_image* in, const tucu_image* out,
                                 int shift, void* stream);
 
+/* Extension (no counterpart in the reference, whose FIRFilter2 lobes are
+ * always symmetric or antisymmetric): a separable filter with general taps.
+ * tapsH / tapsV hold nH / nV (odd, 1..33) taps; tap i of a window starting
+ * at (y, x) weights in[y][x+i] (horizontal) or in[y+i][.] (vertical), the
+ * window convention of FIRFilter2.  With r = (n-1)/2 per direction the
+ * output placement, valid region, no-op rule and dtypes are those of
+ * tucu_fir2_convolve; symmetric taps reproduce FIRFilter2 up to rounding. */
+TUCU_API int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int nV,
+                                     const tucu_image* in, const tucu_image* out,
+                                     int shift, void* stream);
+
 /* ---- FIRGaussianConvolver2 ---------------------------------------------- */
 
 /* detail::lobeSize(lobe, even) (FIRGaussianConvolver.cu:52-98) over a

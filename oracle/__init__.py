@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
                   ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.orc_fir2_f32.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_int] + common
         L.orc_fir2_f64.argtypes = [f64p, ctypes.c_int, f64p, ctypes.c_int] + common
+        L.orc_fir2_taps_f32.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_int] + common
+        L.orc_fir2_taps_f64.argtypes = [f64p, ctypes.c_int, f64p, ctypes.c_int] + common
         L.orc_half_to_float.argtypes = [ctypes.c_uint16]
         L.orc_half_to_float.restype = ctypes.c_float
         L.orc_float_to_half.argtypes = [ctypes.c_float]
@@ -150,6 +152,31 @@ def fir2(lobeH, lobeV, img: np.ndarray, out: np.ndarray, shift: bool = True,
         raise LobeTooLarge("FIRFilter2<T, BLOCK_TRAITS>::initialize: too large lobe size!")
     if rc == 3:
         raise UnsupportedLobe(f"FIRFilter2::convolve: unsupported lobe size [{lh.size}, {lv.size}]")
+    if rc != 0:
+        raise OracleError(f"oracle returned {rc}")
+    return out
+
+
+def fir2_taps(tapsH, tapsV, img: np.ndarray, out: np.ndarray, shift: bool = True,
+              precision: str = "f32", nthreads: int = 1) -> np.ndarray:
+    """General-taps separable filter (extension; FIRFilter2's window
+    convention with odd tap counts up to 33, tap i weighting in[.][x+i])."""
+    assert img.ndim == 2 and out.ndim == 2
+    nrow, ncol = img.shape
+    if out.shape[0] < nrow or out.shape[1] < ncol:
+        raise ValueError("output must be at least as large as the input")
+    if precision == "f32":
+        ct, dt, fn = ctypes.c_float, np.float32, lib().orc_fir2_taps_f32
+    else:
+        ct, dt, fn = ctypes.c_double, np.float64, lib().orc_fir2_taps_f64
+    th = np.ascontiguousarray(tapsH, dtype=dt)
+    tv = np.ascontiguousarray(tapsV, dtype=dt)
+    p = ctypes.POINTER(ct)
+    rc = fn(th.ctypes.data_as(p), th.size, tv.ctypes.data_as(p), tv.size,
+            img.ctypes.data, _dt(img), img.strides[0] // img.itemsize, nrow, ncol,
+            out.ctypes.data, _dt(out), out.strides[0] // out.itemsize, int(shift), int(nthreads))
+    if rc == 3:
+        raise UnsupportedLobe("general taps: counts must be odd, 1 to 33")
     if rc != 0:
         raise OracleError(f"oracle returned {rc}")
     return out

@@ -217,6 +217,42 @@ int fir2(const A* lobeH, int nH, const A* lobeV, int nV,
     return 0;
 }
 
+// General taps (an extension with no counterpart in the reference): the
+// FIRFilter2 window convention with an arbitrary odd number of taps,
+// tap i of the window weighting in[x + i], summed left to right.
+template <class A>
+int fir2_taps(const A* tapsH, int nH, const A* tapsV, int nV,
+              const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+              void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    if (nH < 1 || nV < 1 || !(nH & 1) || !(nV & 1) || nH > 33 || nV > 33) return 3;
+    const int rV = nV / 2, rH = nH / 2;
+    if (nrow <= 2 * rV || ncol <= 2 * rH) return 0;
+    const int ncb = ncol - 2 * rH;
+    std::vector<A> buf(size_t(nrow) * ncb);
+    parallel_rows(nrow, nthreads, [&](int yb, int ye) {
+        std::vector<A> row(ncol);
+        for (int y = yb; y < ye; ++y) {
+            for (int x = 0; x < ncol; ++x) row[x] = load_px<A>(in, in_dt, int64_t(y) * in_stride + x);
+            for (int x = 0; x < ncb; ++x) {
+                A acc = tapsH[0] * row[x];
+                for (int j = 1; j < nH; ++j) acc = acc + tapsH[j] * row[x + j];
+                buf[size_t(y) * ncb + x] = acc;
+            }
+        }
+    });
+    const int oy = shift ? rV : 0, ox = shift ? rH : 0;
+    parallel_rows(nrow - 2 * rV, nthreads, [&](int yb, int ye) {
+        for (int y = yb; y < ye; ++y)
+            for (int x = 0; x < ncb; ++x) {
+                A acc = tapsV[0] * buf[size_t(y) * ncb + x];
+                for (int i = 1; i < nV; ++i) acc = acc + tapsV[i] * buf[size_t(y + i) * ncb + x];
+                store_px<A>(out, out_dt, int64_t(y + oy) * out_stride + (x + ox), acc);
+            }
+    });
+    return 0;
+}
+
 // detail::lobeSize (FIRGaussianConvolver.cu:52-98), generic in the element type.
 template <class A> size_t lobe_size(const A lobe[], bool even)
 {
@@ -309,6 +345,22 @@ int orc_fir2_f64(const double* lobeH, int nH, const double* lobeV, int nV,
 {
     return fir2<double>(lobeH, nH, lobeV, nV, in, in_dt, in_stride, nrow, ncol,
                         out, out_dt, out_stride, shift, nthreads);
+}
+
+int orc_fir2_taps_f32(const float* tapsH, int nH, const float* tapsV, int nV,
+                      const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+                      void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    return fir2_taps<float>(tapsH, nH, tapsV, nV, in, in_dt, in_stride, nrow, ncol,
+                            out, out_dt, out_stride, shift, nthreads);
+}
+
+int orc_fir2_taps_f64(const double* tapsH, int nH, const double* tapsV, int nV,
+                      const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+                      void* out, int out_dt, int64_t out_stride, int shift, int nthreads)
+{
+    return fir2_taps<double>(tapsH, nH, tapsV, nV, in, in_dt, in_stride, nrow, ncol,
+                             out, out_dt, out_stride, shift, nthreads);
 }
 
 // Element conversions, exposed so the tests can pin them against numpy.

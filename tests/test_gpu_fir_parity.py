@@ -117,3 +117,21 @@ def test_fir_strided_views(dev):
     outside = got.copy()
     outside[10:90, 3:303] = sentinel(np.float32)
     assert (outside == sentinel(np.float32)).all()
+
+
+@pytest.mark.parametrize("nH,nV", [(1, 1), (3, 5), (17, 9), (9, 17), (33, 3), (5, 33), (11, 25)])
+def test_general_taps(dev, nH, nV):
+    """General-taps extension against the oracle, every radius class and the
+    edges of the tap-count range; u8 and float inputs, shift both ways."""
+    import tucuda_amd
+    rng = np.random.default_rng(nH * 40 + nV)
+    th = rng.standard_normal(nH)
+    tv = rng.standard_normal(nV)
+    th, tv = (th / np.abs(th).sum()).astype(np.float32), (tv / np.abs(tv).sum()).astype(np.float32)
+    for ik, shift in itertools.product(("u8", "f32"), (True, False)):
+        img = _input(ik, nH + nV, 77, 301)
+        ref = np.full(img.shape, sentinel(np.float32), np.float32)
+        oracle.fir2_taps(th, tv, img, ref, shift)
+        d_out = to_dev(np.full(img.shape, sentinel(np.float32), np.float32), dev)
+        tucuda_amd.SeparableFilter2(th, tv).convolve(to_dev(img, dev), d_out, shift)
+        assert_f32_parity(to_host(d_out), ref, f"taps {nH}x{nV} {ik} shift={shift}")

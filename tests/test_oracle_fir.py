@@ -207,3 +207,30 @@ def test_conversions_match_numpy():
         want = ((bits + 0x7FFF + lsb) >> 16) & 0xFFFF
         assert oracle.float_to_bf16(float(x)) == want, x
         assert oracle.bf16_to_float(want) == float(np.uint32(want << 16).view(np.float32))
+
+
+@pytest.mark.parametrize("nH,nV", [(17, 9), (33, 33), (1, 5), (7, 3), (3, 31)])
+def test_general_taps_brute_force(nH, nV):
+    """The general-taps extension against the 2-D window definition; with
+    symmetric taps it equals FIRFilter2 on the corresponding lobe."""
+    rng = np.random.default_rng(nH * 7 + nV)
+    th, tv = rng.standard_normal(nH), rng.standard_normal(nV)
+    th, tv = (th / np.abs(th).sum()).astype(np.float32), (tv / np.abs(tv).sum()).astype(np.float32)
+    img = synth.uniform_image(3, 50, 60)
+    rH, rV = nH // 2, nV // 2
+    f = img.astype(np.float64)
+    acc = np.zeros((50 - 2 * rV, 60 - 2 * rH))
+    for i in range(nV):
+        for j in range(nH):
+            acc += float(tv[i]) * float(th[j]) * f[i:i + 50 - 2 * rV, j:j + 60 - 2 * rH]
+    out = np.full(img.shape, np.nan, np.float32)
+    oracle.fir2_taps(th, tv, img, out)
+    np.testing.assert_allclose(out[rV:rV + acc.shape[0], rH:rH + acc.shape[1]], acc, rtol=1e-5, atol=1e-4)
+    assert np.isnan(out[:rV]).all() and np.isnan(out[:, :rH]).all()
+    lobe = random_lobe(rng, 9)
+    sym = oracle.full_kernel(lobe).astype(np.float32)
+    a = np.zeros(img.shape, np.float32)
+    b = np.zeros(img.shape, np.float32)
+    oracle.fir2(lobe, lobe, img, a)
+    oracle.fir2_taps(sym, sym, img, b)
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-4)

@@ -14,7 +14,7 @@ from ._capi import (  # noqa: F401
 
 def __getattr__(name):
     # torch is imported lazily so that the ABI can be inspected without it.
-    if name in ("FIRFilter2", "FIRGaussianConvolver2", "gauss_lobes", "lobe_size",
+    if name in ("FIRFilter2", "FIRGaussianConvolver2", "SeparableFilter2", "gauss_lobes", "lobe_size",
                 "METHOD_ORDER", "METHOD_BITS", "METHOD_LOBES", "image_desc"):
         from . import filters
         return getattr(filters, name)

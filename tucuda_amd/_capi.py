@@ -26,6 +26,7 @@ TUCU_ALL6 = 0x3F
 EXPORTS = (
     "tucu_last_error", "tucu_version", "tucu_malloc", "tucu_free", "tucu_row_stride",
     "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
+    "tucu_fir2_convolve_taps",
     "tucu_lobe_size", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
 )
 
@@ -100,6 +101,8 @@ def load() -> ctypes.CDLL:
     L.tucu_fir2_check.argtypes = [ctypes.c_int, ctypes.c_int]
     L.tucu_fir2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, imgp, imgp,
                                      ctypes.c_int, ctypes.c_void_p]
+    L.tucu_fir2_convolve_taps.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, imgp, imgp,
+                                          ctypes.c_int, ctypes.c_void_p]
     L.tucu_lobe_size.argtypes = [fp, ctypes.c_int]
     L.tucu_gauss2_lobes.argtypes = [ctypes.c_float, ctypes.c_int, fp, ip, fp, ip, fp, ip]
     L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,

@@ -19,6 +19,7 @@
 namespace tucu {
 
 KernelEntry g_fir_table[4][4][4][4];
+KernelEntry g_taps_table[4][4][4][4];
 KernelEntry g_gauss_table[5][NGTH][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
@@ -268,6 +269,39 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
     if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
     if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
     const KernelEntry& k = g_fir_table[fir_class(rV)][fir_class(rH)][in->dtype][out->dtype];
+    return launch(k, a, static_cast<hipStream_t>(stream));
+}
+
+int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int nV,
+                            const tucu_image* in, const tucu_image* out, int shift, void* stream)
+{
+    int rc;
+    if (nH < 1 || nV < 1 || nH > TMAX || nV > TMAX || !(nH & 1) || !(nV & 1))
+        return fail(TUCU_ERR_UNSUPPORTED_LOBE, "general taps: counts must be odd, 1 to 33");
+    if ((rc = check_image(in, "input", false))) return rc;
+    if ((rc = check_image(out, "output", false))) return rc;
+    const int nrow = in->nrow, ncol = in->ncol;
+    const int rV = nV / 2, rH = nH / 2;
+    if (nrow <= 2 * rV || ncol <= 2 * rH) return TUCU_OK;
+    if (!tapsH || !tapsV) return fail(TUCU_ERR_INVALID, "null taps");
+    if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
+    if ((rc = check_extent(*out, rV, rH, nrow, ncol, shift != 0, "output"))) return rc;
+    // template radius: the smallest class that holds the taps (a centred
+    // 1-tap filter still uses the radius-2 kernel)
+    auto cls = [](int r) { return r <= 2 ? 0 : r <= 4 ? 1 : r <= 8 ? 2 : 3; };
+    const int RV = FIR_RADII[cls(rV)], RH = FIR_RADII[cls(rH)];
+
+    SepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.cy0 = rV; a.cy1 = nrow - rV;
+    a.cx0 = rH; a.cx1 = ncol - rH;
+    a.mask = 1;
+    a.r[0] = rV; a.r[1] = rH;
+    for (int i = 0; i < nV; ++i) a.tv[RV - rV + i] = tapsV[i];
+    for (int i = 0; i < nH; ++i) a.th[RH - rH + i] = tapsH[i];
+    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
+    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
+    const KernelEntry& k = g_taps_table[cls(rV)][cls(rH)][in->dtype][out->dtype];
     return launch(k, a, static_cast<hipStream_t>(stream));
 }
 

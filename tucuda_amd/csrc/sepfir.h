@@ -36,7 +36,8 @@
 
 namespace tucu {
 
-enum { MODE_FIR = 0, MODE_GAUSS = 1 };
+enum { MODE_FIR = 0, MODE_GAUSS = 1, MODE_TAPS = 2 };
+constexpr int TMAX = 33;      // full taps, radius <= 16
 constexpr int NT = 256;        // threads per workgroup = input columns per tile
 constexpr int KMAX = 17;       // centre-indexed coefficients, radius <= 16
 constexpr int BUF_DWORD3 = 0x00020000;   // raw buffer resource word 3 on gfx9-family parts
@@ -61,6 +62,7 @@ struct SepArgs {
     float    sgnv, sgnh;           // FIR: +1 symmetric lobe, -1 antisymmetric lobe
     float    cv[3][KMAX];          // vertical lobes (centre-indexed, zero padded); GAUSS: both passes
     float    ch[KMAX];             // FIR: horizontal lobe
+    float    tv[TMAX], th[TMAX];   // TAPS: full vertical / horizontal taps, tap k at offset k - R
     uintptr_t obase[6];            // output k: address of the element for centre (0, 0), i.e.
                                    // data + (dy_k * stride + dx_k) * elem (may lie before data)
     int32_t  ostride[6];           // output row stride, elements
@@ -257,13 +259,30 @@ vpass_anti(const SepArgs& a, const float* w, float* lds, int t)
     for (int i = 0; i < TH; ++i) lds[(1 * TH + i) * PITCH + t] = acc[i];
 }
 
+// Vertical pass with general taps (no symmetry): field 0 = sum_k tv[k] *
+// w[y - RV + k], k = 0 .. 2 RV, top tap first.
+template <int RV, int TH, int PITCH>
+__device__ __forceinline__ void
+vpass_taps(const SepArgs& a, const float* w, float* lds, int t)
+{
+    float acc[TH];
+#pragma unroll
+    for (int k = 0; k <= 2 * RV; ++k) {
+#pragma unroll
+        for (int i = 0; i < TH; ++i)
+            acc[i] = k == 0 ? a.tv[0] * w[i] : __builtin_fmaf(a.tv[k], w[i + k], acc[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < TH; ++i) lds[i * PITCH + t] = acc[i];
+}
+
 // Horizontal pass of field F over the tile: every output the field feeds.
 //   GAUSS field 0 -> smooth (lobe 0), diffH (lobe 1), diffHH (lobe 2)
 //         field 1 -> diffV (lobe 0), diffHV (lobe 1)
 //         field 2 -> diffVV (lobe 0)
 //   FIR   field 0 -> output 0 (lobe ch, runtime parity sgnh)
 // FAST: every store of the tile is valid and aligned (no checks).
-template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, bool FAST, class OUT>
+template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, bool FAST, class OUT, bool TAPS = false>
 __device__ __forceinline__ void
 hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
 {
@@ -296,9 +315,17 @@ hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
             const float4 q = row[m];
             v[4 * m] = q.x; v[4 * m + 1] = q.y; v[4 * m + 2] = q.z; v[4 * m + 3] = q.w;
         }
+        float r0[4], r1[4], r2[4];
+        if constexpr (TAPS) {
+            // general taps, left tap first
+#pragma unroll
+            for (int k = 0; k <= 2 * RH; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    r0[q] = k == 0 ? a.th[0] * v[q] : __builtin_fmaf(a.th[k], v[q + k], r0[q]);
+        } else {
         // Tap-outer: 4 columns x up to 3 lobes of running sums, each summed
         // over k = RH .. 1 then the centre (the order sym_sum uses).
-        float r0[4], r1[4], r2[4];
         const float* c0 = FIR ? a.ch : a.cv[0];
 #pragma unroll
         for (int k = RH; k >= 1; --k) {
@@ -322,25 +349,27 @@ hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
             r0[q] = __builtin_fmaf(c0[0], v[q + RH], r0[q]);
             if (H2) r2[q] = __builtin_fmaf(a.cv[2][0], v[q + RH], r2[q]);
         }
+        }
         if (st0) store4<OUT>(a, O0, FAST, vy0, vy1, rh0, a.ncol - rh0, y, xc, r0[0], r0[1], r0[2], r0[3]);
         if (H1 && st1) store4<OUT>(a, O1, FAST, vy0, vy1, a.r[1], a.ncol - a.r[1], y, xc, r1[0], r1[1], r1[2], r1[3]);
         if (H2 && st2) store4<OUT>(a, O2, FAST, vy0, vy1, a.r[2], a.ncol - a.r[2], y, xc, r2[0], r2[1], r2[2], r2[3]);
     }
 }
 
-template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, class OUT>
+template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, class OUT, bool TAPS = false>
 __device__ __forceinline__ void
 hpass(const SepArgs& a, const float* lds, int t, int cy, int cx, bool fast)
 {
-    if (fast) hpass_field<RH, TH, PITCH, TWO, F, FIR, true, OUT>(a, lds, t, cy, cx);
-    else      hpass_field<RH, TH, PITCH, TWO, F, FIR, false, OUT>(a, lds, t, cy, cx);
+    if (fast) hpass_field<RH, TH, PITCH, TWO, F, FIR, true, OUT, TAPS>(a, lds, t, cy, cx);
+    else      hpass_field<RH, TH, PITCH, TWO, F, FIR, false, OUT, TAPS>(a, lds, t, cy, cx);
 }
 
 template <int RV, int RH, int TH, int MODE, class IN, class OUT>
 __global__ void __launch_bounds__(NT)
 sepfir_kernel(const SepArgs a)
 {
-    constexpr bool FIR  = MODE == MODE_FIR;
+    constexpr bool TAPS = MODE == MODE_TAPS;
+    constexpr bool FIR  = MODE == MODE_FIR || TAPS;     // one field, one output
     constexpr int NF    = FIR ? 1 : 3;
     constexpr int TWO   = NT - 2 * RH;          // centre columns per tile
     constexpr int PITCH = NT + 4;               // LDS row pitch (floats), keeps 16-byte rows
@@ -370,7 +399,9 @@ sepfir_kernel(const SepArgs a)
             __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
         float w[NWIN];
         load_column<NWIN, IN>(a, rsrc, cx - RH + t, cy - RV, w);
-        if constexpr (FIR) {
+        if constexpr (TAPS) {
+            vpass_taps<RV, TH, PITCH>(a, w, lds, t);
+        } else if constexpr (FIR) {
             vpass_sym<RV, TH, PITCH, true, false, true>(a, w, lds, t);
         } else {
             if (needF0 && needF2)  vpass_sym<RV, TH, PITCH, true, true, false>(a, w, lds, t);
@@ -385,7 +416,7 @@ sepfir_kernel(const SepArgs a)
     // Uniform: is every store of this tile valid and vector-aligned?
     const bool fast = cy >= a.fy0 && cy + TH <= a.fy1 && cx >= a.fx0 && cx + TWO <= a.fx1;
     if constexpr (FIR) {
-        hpass<RH, TH, PITCH, TWO, 0, true, OUT>(a, lds, t, cy, cx, fast);
+        hpass<RH, TH, PITCH, TWO, 0, true, OUT, TAPS>(a, lds, t, cy, cx, fast);
     } else {
         if (needF0) hpass<RH, TH, PITCH, TWO, 0, false, OUT>(a, lds, t, cy, cx, fast);
         if (needF1) hpass<RH, TH, PITCH, TWO, 1, false, OUT>(a, lds, t, cy, cx, fast);
@@ -423,6 +454,7 @@ constexpr int GAUSS_TH_DEFAULT = 2;       // index into GAUSS_THS (8 rows)
 // (the tuned paths); other pairs carry the default height only, and the
 // host falls back to it.
 extern KernelEntry g_fir_table[4][4][4][4];
+extern KernelEntry g_taps_table[4][4][4][4];     // general taps, same layout as g_fir_table
 extern KernelEntry g_gauss_table[5][NGTH][4][4];
 
 }   // namespace tucu

@@ -105,6 +105,32 @@ class FIRFilter2:
         return self._lobeV.size & ~1
 
 
+class SeparableFilter2:
+    """Extension: separable filter with general (non-symmetric) taps, the
+    window convention of FIRFilter2 (tap i weights in[y][x+i] / in[y+i][x]).
+    Odd tap counts 1..33; output placement as FIRFilter2 with r = (n-1)/2."""
+
+    def __init__(self, tapsH=None, tapsV=None):
+        self._th = _f32(tapsH if tapsH is not None else [1.0])
+        self._tv = _f32(tapsV if tapsV is not None else [1.0])
+
+    def initialize(self, tapsH, tapsV) -> "SeparableFilter2":
+        self._th, self._tv = _f32(tapsH), _f32(tapsV)
+        return self
+
+    def offsetH(self) -> int:
+        return self._th.size // 2
+
+    def offsetV(self) -> int:
+        return self._tv.size // 2
+
+    def convolve(self, inp: torch.Tensor, out: torch.Tensor, shift: bool = True) -> None:
+        src, dst = image_desc(inp), image_desc(out)
+        check(_capi.load().tucu_fir2_convolve_taps(
+            _fptr(self._th), self._th.size, _fptr(self._tv), self._tv.size,
+            ctypes.byref(src), ctypes.byref(dst), int(bool(shift)), _stream_handle(inp)))
+
+
 def gauss_lobes(sigma: float, rule: int = _capi.TUCU_RULE_REFERENCE):
     """FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)
     (FIRGaussianConvolver.h:123-170), computed by the library on the host."""
