@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--sets", type=int, default=10, help="rotating buffer sets (>= 5 defeats the 256 MB MALL)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--persistent", type=int, default=0, metavar="ROWS",
+                    help="run each image through the persistent batched kernel with ROWS-row tiles "
+                         "(4, 8 or 16) instead of one workgroup per tile")
     ap.add_argument("--eager", action="store_true",
                     help="launch every step from Python instead of replaying a captured HIP graph")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -110,6 +113,25 @@ def main():
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream(dev)
+    if args.persistent:
+        batches = []
+        for ins, outs in sets:
+            b = tucuda_amd.GaussBatch([(ins, outs, g._lobes())])
+            b.set_tile_rows(args.persistent)
+            batches.append(b)
+
+        class _One:
+            def __init__(self, b):
+                self.b = b
+
+            def convolve6(self, ins, outs):
+                self.b.launch()
+        by_set = {id(s[0]): _One(b) for s, b in zip(sets, batches)}
+
+        class _Dispatch:
+            def convolve6(self, ins, outs):
+                by_set[id(ins)].convolve6(ins, outs)
+        g = _Dispatch()
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -208,6 +230,7 @@ def main():
             "nrow": nrow, "ncol": ncol, "sigma": SIGMA, "outputs": 6,
             "buffer_sets": args.sets, "shift": True,
             "launch": "eager" if args.eager else "hip-graph replay",
+            "kernel": f"persistent, {args.persistent}-row tiles" if args.persistent else "one workgroup per tile",
         },
         "roofline": {
             "bound": "hbm",

@@ -161,6 +161,47 @@ TUCU_API int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1
                                   const tucu_image* in, const tucu_image* outs,
                                   unsigned mask, int shift, void* stream);
 
+/* ---- batched mode (extension) ------------------------------------------
+ * Many FIRGaussianConvolver2 jobs of uneven sizes and lobe radii in one
+ * persistent launch.  No counterpart in the reference, which filters one
+ * image per call (FIRGaussianConvolver.h:178-242); each job obeys the
+ * contract of tucu_gauss2_convolve and produces exactly what that call
+ * produces for it.  The launch allocates nothing and can be captured in a
+ * HIP graph; a batch object must not be launched concurrently with itself
+ * (it owns the launch's tile counters). */
+
+typedef struct tucu_batch_s* tucu_batch;
+
+typedef struct {
+    tucu_image   in;
+    tucu_image   out[6];      /* destination of method k (TUCU_SMOOTH .. TUCU_DIFFVV order) */
+    const float* lobe0;       /* lobes as for tucu_gauss2_convolve */
+    const float* lobe1;
+    const float* lobe2;
+    int32_t      n0, n1, n2;
+    uint32_t     mask;        /* outputs wanted */
+    int32_t      shift;
+} tucu_gauss_job;
+
+/* Validates every job, copies its arguments and lays out the device tile
+ * tables (allocates device memory).  All jobs share input and output
+ * element types. */
+TUCU_API int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs);
+TUCU_API int tucu_batch_destroy(tucu_batch batch);
+/* Enqueue the whole batch on `stream` (one kernel launch). */
+TUCU_API int tucu_batch_launch(tucu_batch batch, void* stream);
+/* Tile height in rows (4, 8 or 16 where instantiated; 0 = the tuned value
+ * for batches of this shape if the tuning cache has one, else 8). */
+TUCU_API int tucu_batch_set_tile_rows(tucu_batch batch, int rows);
+/* Times `reps` launches per available tile height on `stream` (the outputs
+ * are written), keeps the fastest for this batch and records it in the
+ * process-wide tuning cache under (dtypes, largest radius, log2 pixels). */
+TUCU_API int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows, float* best_us);
+TUCU_API int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* tile_rows);
+/* Tuning cache as JSON lines. */
+TUCU_API int tucu_tune_cache_save(const char* path);
+TUCU_API int tucu_tune_cache_load(const char* path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,8 @@ EXPORTS = (
     "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
     "tucu_fir2_convolve_taps",
     "tucu_lobe_size", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
+    "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
+    "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
 )
 
 
@@ -39,6 +41,21 @@ class TucuImage(ctypes.Structure):
         ("ncol", ctypes.c_int32),
         ("dtype", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+    ]
+
+
+class TucuGaussJob(ctypes.Structure):
+    _fields_ = [
+        ("in_", TucuImage),
+        ("out", TucuImage * 6),
+        ("lobe0", ctypes.POINTER(ctypes.c_float)),
+        ("lobe1", ctypes.POINTER(ctypes.c_float)),
+        ("lobe2", ctypes.POINTER(ctypes.c_float)),
+        ("n0", ctypes.c_int32),
+        ("n1", ctypes.c_int32),
+        ("n2", ctypes.c_int32),
+        ("mask", ctypes.c_uint32),
+        ("shift", ctypes.c_int32),
     ]
 
 
@@ -107,6 +124,15 @@ def load() -> ctypes.CDLL:
     L.tucu_gauss2_lobes.argtypes = [ctypes.c_float, ctypes.c_int, fp, ip, fp, ip, fp, ip]
     L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,
                                        imgp, imgp, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]
+    vp = ctypes.c_void_p
+    L.tucu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TucuGaussJob), ctypes.c_int]
+    L.tucu_batch_destroy.argtypes = [vp]
+    L.tucu_batch_launch.argtypes = [vp, vp]
+    L.tucu_batch_set_tile_rows.argtypes = [vp, ctypes.c_int]
+    L.tucu_batch_autotune.argtypes = [vp, vp, ctypes.c_int, ip, fp]
+    L.tucu_batch_info.argtypes = [vp, ip, ip, ip, ip]
+    L.tucu_tune_cache_save.argtypes = [ctypes.c_char_p]
+    L.tucu_tune_cache_load.argtypes = [ctypes.c_char_p]
     for name in EXPORTS:
         if name != "tucu_last_error" and name != "tucu_row_stride" and name != "tucu_version":
             getattr(L, name).restype = ctypes.c_int

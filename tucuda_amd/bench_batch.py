@@ -1,0 +1,135 @@
+"""Batched-mode measurement: 256 images of uneven sizes in one launch.
+
+    python -m tucuda_amd.bench_batch [--images 256] [--reps 20] [--seed 3]
+
+Workload: widths and heights log-uniform over [240, 2160] with the aspect
+ratio clamped to [0.5, 2], per-image sigma uniform in [0.8, 4.0], lobe
+radius ceil(4 sigma) (TUCU_RULE_CEIL4SIGMA), all six outputs; float32 and
+bfloat16 storage.  Compared forms, all on the same resident buffers:
+
+* ``plain``: one fused six-output launch per image (the single-image API);
+* ``per_method``: six FIRGaussianConvolver2 method calls per image, the way
+  the reference's API is used (FIRGaussianConvolver.h:178-242); reference-
+  rule lobes, since FIRFilter2 only takes the reference's lobe lengths;
+* ``batched``: one persistent launch over every image, tile height picked by
+  the autotuner, replayed from a captured HIP graph.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+
+def make_sizes(n: int, seed: int):
+    from . import synth
+    u = synth.uniform01(seed, 3 * n).reshape(n, 3)
+    lo, hi = math.log(240), math.log(2160)
+    out = []
+    for a, b, c in u:
+        h = int(round(math.exp(lo + (hi - lo) * a)))
+        w = int(round(math.exp(lo + (hi - lo) * b)))
+        w = int(min(max(w, h / 2), 2 * h))
+        sigma = float(np.float32(0.8 + 3.2 * c))
+        out.append((h, w, sigma))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--images", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--dtypes", default="float32,bfloat16")
+    args = ap.parse_args(argv)
+
+    import torch
+    import tucuda_amd
+    from . import synth
+
+    dev = torch.device("cuda:0")
+    sizes = make_sizes(args.images, args.seed)
+    total_px = sum(h * w for h, w, _ in sizes)
+    result = {"workload": f"{args.images} images, log-uniform [240,2160], sigma U[0.8,4], radius ceil(4 sigma), 6 outputs",
+              "images": args.images, "megapixels": round(total_px / 1e6, 2), "forms": {}}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    for dname in args.dtypes.split(","):
+        dt = getattr(torch, dname)
+        ins, outs = [], []
+        for i, (h, w, _) in enumerate(sizes):
+            img = synth.edge_image(1000 + i, h, w)
+            ins.append(torch.from_numpy(img).to(dev).to(dt))
+            outs.append([torch.empty((h, w), dtype=dt, device=dev) for _ in range(6)])
+        gs = [tucuda_amd.FIRGaussianConvolver2(s, rule=tucuda_amd.TUCU_RULE_CEIL4SIGMA) for _, _, s in sizes]
+
+        def plain():
+            for g, x, o in zip(gs, ins, outs):
+                g.convolve6(x, o)
+
+        ms_plain = timed(plain, max(1, args.reps // 4))
+
+        t0 = time.perf_counter()
+        batch = tucuda_amd.GaussBatch([(x, o, g._lobes()) for g, x, o in zip(gs, ins, outs)])
+        t_build = time.perf_counter() - t0
+        rows, tuned_us = batch.autotune(reps=5)
+        graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(dev)
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(graph, stream=s):
+                batch.launch()
+        torch.cuda.current_stream(dev).wait_stream(s)
+        ms_batch = timed(graph.replay, args.reps)
+        ms_batch_eager = timed(batch.launch, args.reps)
+        form = {
+            "plain_ms": round(ms_plain, 3),
+            "batched_ms": round(ms_batch, 3),
+            "batched_eager_ms": round(ms_batch_eager, 3),
+            "plain_gpix_s": round(total_px / ms_plain / 1e6, 2),
+            "batched_gpix_s": round(total_px / ms_batch / 1e6, 2),
+            "speedup_vs_plain": round(ms_plain / ms_batch, 2),
+            "tile_rows": rows, "tiles": batch.info()["tiles"], "grid": batch.info()["grid"],
+            "table_build_s": round(t_build, 3),
+        }
+        if dname == "float32":
+            ref_gs = [tucuda_amd.FIRGaussianConvolver2(sg) for _, _, sg in sizes]
+
+            def per_method():
+                for g, x, o in zip(ref_gs, ins, outs):
+                    g.smooth(x, o[0]); g.diffH(x, o[1]); g.diffV(x, o[2])
+                    g.diffHH(x, o[3]); g.diffHV(x, o[4]); g.diffVV(x, o[5])
+
+            ms_pm = timed(per_method, max(1, args.reps // 4))
+            form["per_method_ms"] = round(ms_pm, 3)
+            form["per_method_gpix_s"] = round(total_px / ms_pm / 1e6, 2)
+        result["forms"][dname] = form
+        del batch, graph, ins, outs
+        torch.cuda.empty_cache()
+    f32 = result["forms"].get("float32", {})
+    bf = result["forms"].get("bfloat16", {})
+    if f32 and bf:
+        result["bf16_batched_vs_f32_plain"] = round(f32["plain_ms"] / bf["batched_ms"], 2)
+        if "per_method_ms" in f32:
+            result["bf16_batched_vs_f32_per_method"] = round(f32["per_method_ms"] / bf["batched_ms"], 2)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,305 @@
+// batch_host.hip -- batched mode of the C ABI: many Gaussian jobs of uneven
+// sizes and radii in one persistent launch, with a tile-height autotuner.
+#include "hostutil.h"
+#include "batch.h"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace tucu {
+
+BatchEntry g_batch_table[NBTH][4][4];
+
+namespace {
+
+using namespace host;
+
+// Device copy of a batch laid out for one tile height.
+struct Layout {
+    SepArgs*   d_jobs  = nullptr;
+    BatchTile* d_tiles = nullptr;
+    int        ntiles  = 0;
+    int        grid    = 0;
+    bool       built   = false;
+};
+
+// Tuning cache: (in dtype, out dtype, radius class of the largest job,
+// log2 of the tile count at the default height) -> best tile-height index.
+using TuneKey = std::tuple<int, int, int, int>;
+struct TuneVal { int th_index; float us; };
+std::mutex                  g_tune_mutex;
+std::map<TuneKey, TuneVal>  g_tune;
+
+}   // namespace
+}   // namespace tucu
+
+using namespace tucu;
+using namespace tucu::host;
+
+struct tucu_batch_s {
+    int                  in_dt = 0, out_dt = 0;
+    std::vector<SepArgs> jobs;
+    Layout               layout[NBTH];
+    int*                 d_counter = nullptr;    // tile counter of the launches (cleared by each)
+    int                  th_index = -1;          // -1: tuned value if cached, else default
+    int                  active = BATCH_TH_DEFAULT;   // layout the next launch uses (always built)
+    int                  rclass = 0;
+    int                  bucket = 0;
+};
+
+namespace {
+
+int free_layout(Layout& l)
+{
+    if (l.d_jobs) (void)hipFree(l.d_jobs);
+    if (l.d_tiles) (void)hipFree(l.d_tiles);
+    l = Layout();
+    return TUCU_OK;
+}
+
+// Tile grid of every job for tile height th, tiles of the larger radius
+// classes first (they cost most; handing them out early shortens the tail).
+int build_layout(tucu_batch_s& b, int thi)
+{
+    Layout& l = b.layout[thi];
+    if (l.built) return TUCU_OK;
+    const BatchEntry& e = g_batch_table[thi][b.in_dt][b.out_dt];
+    if (!e.fn) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
+    const int th = e.th;
+    std::vector<SepArgs> jobs = b.jobs;
+    for (SepArgs& a : jobs) a.tiles_y = (a.cy1 - a.cy0 + th - 1) / th;
+    std::vector<int> order(jobs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].rt > jobs[y].rt; });
+    std::vector<BatchTile> all;
+    for (int j : order) {
+        const SepArgs& a = jobs[j];
+        const int two = NT - 2 * a.rt;
+        const int64_t n = int64_t((a.cx1 - a.cx0 + two - 1) / two) * a.tiles_y;
+        for (int64_t t = 0; t < n; ++t) all.push_back({uint32_t(j), uint32_t(t)});
+    }
+    if (all.size() >= size_t(INT32_MAX)) return fail(TUCU_ERR_INVALID, "batch too large");
+    hipError_t err;
+    if ((err = hipMalloc(&l.d_jobs, jobs.size() * sizeof(SepArgs))) != hipSuccess ||
+        (err = hipMalloc(&l.d_tiles, std::max<size_t>(1, all.size()) * sizeof(BatchTile))) != hipSuccess) {
+        free_layout(l);
+        (void)hipGetLastError();
+        return fail(TUCU_ERR_BAD_ALLOC, "device allocation for the batch tables failed");
+    }
+    if ((err = hipMemcpy(l.d_jobs, jobs.data(), jobs.size() * sizeof(SepArgs), hipMemcpyHostToDevice)) != hipSuccess ||
+        (all.size() && (err = hipMemcpy(l.d_tiles, all.data(), all.size() * sizeof(BatchTile),
+                                        hipMemcpyHostToDevice)) != hipSuccess)) {
+        free_layout(l);
+        return device_fail(err, "hipMemcpy");
+    }
+    int dev = 0, ncu = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, NT, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    l.ntiles = int(all.size());
+    l.grid = std::max(1, std::min(l.ntiles, std::max(1, ncu) * per_cu));
+    l.built = true;
+    return TUCU_OK;
+}
+
+int pick_th(const tucu_batch_s& b)
+{
+    if (b.th_index >= 0) return b.th_index;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    const auto it = g_tune.find(TuneKey(b.in_dt, b.out_dt, b.rclass, b.bucket));
+    return it != g_tune.end() ? it->second.th_index : BATCH_TH_DEFAULT;
+}
+
+int launch_layout(tucu_batch_s& b, int thi, hipStream_t s)
+{
+    int rc;
+    if ((rc = build_layout(b, thi))) return rc;
+    const Layout& l = b.layout[thi];
+    if (l.ntiles == 0) return TUCU_OK;
+    const hipError_t e = g_batch_table[thi][b.in_dt][b.out_dt].fn(l.d_jobs, l.d_tiles, l.ntiles,
+                                                                   b.d_counter, l.grid, s);
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "batched kernel launch");
+}
+
+}   // namespace
+
+extern "C" {
+
+int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
+{
+    if (!batch) return fail(TUCU_ERR_INVALID, "null batch handle");
+    *batch = nullptr;
+    if (njobs < 0 || (njobs > 0 && !jobs)) return fail(TUCU_ERR_INVALID, "bad job array");
+    auto b = new tucu_batch_s();
+    int rmax_all = 0;
+    int64_t work = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const tucu_gauss_job& g = jobs[j];
+        const float* lobes[3] = {g.lobe0, g.lobe1, g.lobe2};
+        const int n[3] = {g.n0, g.n1, g.n2};
+        SepArgs a;
+        int rmax = 0, odt = 0;
+        const int rc = gauss_args(lobes, n, &g.in, g.out, g.mask, g.shift, a, rmax, odt);
+        if (rc) { delete b; return rc; }
+        if (!a.mask) continue;                                  // nothing to write for this job
+        if (b->jobs.empty()) { b->in_dt = g.in.dtype; b->out_dt = odt; }
+        else if (g.in.dtype != b->in_dt || odt != b->out_dt) {
+            delete b;
+            return fail(TUCU_ERR_INVALID, "all jobs of a batch must share input and output element types");
+        }
+        b->jobs.push_back(a);
+        rmax_all = std::max(rmax_all, a.rt);
+        work += int64_t(a.cy1 - a.cy0) * (a.cx1 - a.cx0);
+    }
+    b->rclass = rmax_all;
+    b->bucket = work > 0 ? int(std::log2(double(work))) : 0;
+    if (hipMalloc(&b->d_counter, 128) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return fail(TUCU_ERR_BAD_ALLOC, "device allocation for the batch counter failed");
+    }
+    if (!b->jobs.empty()) {
+        b->active = pick_th(*b);
+        const int rc = build_layout(*b, b->active);
+        if (rc) { tucu_batch_destroy(b); return rc; }
+    }
+    *batch = b;
+    return TUCU_OK;
+}
+
+int tucu_batch_destroy(tucu_batch batch)
+{
+    if (!batch) return TUCU_OK;
+    for (Layout& l : batch->layout) free_layout(l);
+    if (batch->d_counter) (void)hipFree(batch->d_counter);
+    delete batch;
+    return TUCU_OK;
+}
+
+int tucu_batch_set_tile_rows(tucu_batch batch, int rows)
+{
+    if (!batch) return fail(TUCU_ERR_INVALID, "null batch");
+    int thi = -1;
+    if (rows == 0) {
+        batch->th_index = -1;
+        thi = pick_th(*batch);
+    } else {
+        for (int i = 0; i < NBTH; ++i)
+            if (BATCH_THS[i] == rows && g_batch_table[i][batch->in_dt][batch->out_dt].fn) thi = i;
+        batch->th_index = thi;
+    }
+    if (thi >= 0) {
+        if (batch->jobs.empty()) return TUCU_OK;
+        const int rc = build_layout(*batch, thi);
+        if (rc == TUCU_OK) batch->active = thi;
+        return rc;
+    }
+    return fail(TUCU_ERR_INVALID, "tile height not available for this type pair");
+}
+
+int tucu_batch_launch(tucu_batch batch, void* stream)
+{
+    if (!batch) return fail(TUCU_ERR_INVALID, "null batch");
+    if (batch->jobs.empty()) return TUCU_OK;
+    return launch_layout(*batch, batch->active, static_cast<hipStream_t>(stream));
+}
+
+int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* tile_rows)
+{
+    if (!batch) return fail(TUCU_ERR_INVALID, "null batch");
+    const int thi = batch->active;
+    const Layout& l = batch->layout[thi];
+    if (njobs) *njobs = int(batch->jobs.size());
+    if (ntiles) *ntiles = l.ntiles;
+    if (grid) *grid = l.grid;
+    if (tile_rows) *tile_rows = BATCH_THS[thi];
+    return TUCU_OK;
+}
+
+int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows, float* best_us)
+{
+    if (!batch) return fail(TUCU_ERR_INVALID, "null batch");
+    if (batch->jobs.empty()) return TUCU_OK;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    reps = std::max(1, reps);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return device_fail(hipGetLastError(), "hipEventCreate");
+    int best = -1;
+    float best_t = 0.f;
+    int rc = TUCU_OK;
+    for (int i = 0; i < NBTH && rc == TUCU_OK; ++i) {
+        if (!g_batch_table[i][batch->in_dt][batch->out_dt].fn) continue;
+        if ((rc = launch_layout(*batch, i, s))) break;                  // builds and warms up
+        std::vector<float> t(reps);
+        for (int r = 0; r < reps && rc == TUCU_OK; ++r) {
+            (void)hipEventRecord(e0, s);
+            rc = launch_layout(*batch, i, s);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = device_fail(hipGetLastError(), "hipEventSynchronize");
+            (void)hipEventElapsedTime(&t[r], e0, e1);
+        }
+        if (rc) break;
+        std::nth_element(t.begin(), t.begin() + reps / 2, t.end());
+        const float med = t[reps / 2] * 1000.f;
+        if (best < 0 || med < best_t) { best = i; best_t = med; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    batch->th_index = -1;
+    batch->active = best;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune[TuneKey(batch->in_dt, batch->out_dt, batch->rclass, batch->bucket)] = {best, best_t};
+    }
+    if (best_rows) *best_rows = BATCH_THS[best];
+    if (best_us) *best_us = best_t;
+    return TUCU_OK;
+}
+
+int tucu_tune_cache_save(const char* path)
+{
+    if (!path) return fail(TUCU_ERR_INVALID, "null path");
+    std::ofstream f(path);
+    if (!f) return fail(TUCU_ERR_INVALID, std::string("cannot write ") + path);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    for (const auto& kv : g_tune)
+        f << "{\"in\": " << std::get<0>(kv.first) << ", \"out\": " << std::get<1>(kv.first)
+          << ", \"radius\": " << std::get<2>(kv.first) << ", \"log2_pixels\": " << std::get<3>(kv.first)
+          << ", \"tile_rows\": " << BATCH_THS[kv.second.th_index] << ", \"us\": " << kv.second.us << "}\n";
+    return TUCU_OK;
+}
+
+int tucu_tune_cache_load(const char* path)
+{
+    if (!path) return fail(TUCU_ERR_INVALID, "null path");
+    std::ifstream f(path);
+    if (!f) return fail(TUCU_ERR_INVALID, std::string("cannot read ") + path);
+    std::string line;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    while (std::getline(f, line)) {
+        int in = -1, out = -1, r = -1, bucket = -1, rows = -1;
+        float us = 0.f;
+        if (std::sscanf(line.c_str(), "{\"in\": %d, \"out\": %d, \"radius\": %d, \"log2_pixels\": %d, "
+                                      "\"tile_rows\": %d, \"us\": %f}", &in, &out, &r, &bucket, &rows, &us) != 6)
+            continue;
+        for (int i = 0; i < NBTH; ++i)
+            if (BATCH_THS[i] == rows && in >= 0 && in < 4 && out >= 0 && out < 4 && g_batch_table[i][in][out].fn)
+                g_tune[TuneKey(in, out, r, bucket)] = {i, us};
+    }
+    return TUCU_OK;
+}
+
+}   // extern "C"

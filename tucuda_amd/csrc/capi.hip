@@ -26,7 +26,7 @@ size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
 void gauss_lobes_ceil4(float, float*, int*, float*, int*, float*, int*);
 
-namespace {
+namespace host {
 
 thread_local std::string t_error;
 
@@ -167,10 +167,79 @@ int launch(const KernelEntry& k, SepArgs& a, hipStream_t s)
     return TUCU_OK;
 }
 
-}   // namespace
+// Validated kernel arguments of a fused Gaussian call: lobes (roles fix the
+// parities: l0, l2 symmetric with r+1 entries, l1 antisymmetric with r),
+// outputs selected by `mask`.  On success a.mask holds the outputs that will
+// be written (0: nothing to do), rmax the largest radius in use and odt the
+// common output dtype.
+int gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in,
+               const tucu_image* outs, unsigned mask, int shift, SepArgs& a, int& rmax, int& odt)
+{
+    int rc;
+    std::memset(&a, 0, sizeof(a));
+    if ((rc = check_image(in, "input", false))) return rc;
+    if (mask & ~unsigned(TUCU_ALL6)) return fail(TUCU_ERR_INVALID, "unknown output bits in mask");
+    if (mask == 0) return TUCU_OK;
+    if (!outs) return fail(TUCU_ERR_INVALID, "null output array");
+    const int r[3] = {n[0] - 1, n[1], n[2] - 1};
+    static const int HV[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};   // (H lobe, V lobe)
+    const int nrow = in->nrow, ncol = in->ncol;
+
+    unsigned live = 0;
+    int cy0 = nrow, cy1 = 0, cx0 = ncol, cx1 = 0;
+    rmax = 0;
+    for (int k = 0; k < 6; ++k) {
+        if (!(mask & (1u << k))) continue;
+        const int h = HV[k][0], v = HV[k][1];
+        for (int l : {h, v}) {
+            if (r[l] < 1 || r[l] > 16)
+                return fail(TUCU_ERR_UNSUPPORTED_LOBE, "lobe " + std::to_string(l) + " length outside the supported range");
+            if (!lobes[l]) return fail(TUCU_ERR_INVALID, "null lobe");
+        }
+        const int rv = r[v], rh = r[h];
+        if (nrow <= 2 * rv || ncol <= 2 * rh) continue;               // FIRFilter.h:304-312
+        const std::string name = "output " + std::to_string(k);
+        if ((rc = check_image(&outs[k], name.c_str(), true))) return rc;
+        if ((rc = check_extent(outs[k], rv, rh, nrow, ncol, shift != 0, name.c_str()))) return rc;
+        live |= 1u << k;
+        rmax = std::max(rmax, std::max(rv, rh));
+        cy0 = std::min(cy0, rv); cy1 = std::max(cy1, nrow - rv);
+        cx0 = std::min(cx0, rh); cx1 = std::max(cx1, ncol - rh);
+    }
+    if (!live) return TUCU_OK;
+    if (!in->data) return fail(TUCU_ERR_INVALID, "input: null data");
+    odt = -1;
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        if (odt < 0) odt = outs[k].dtype;
+        else if (outs[k].dtype != odt)
+            return fail(TUCU_ERR_INVALID, "all outputs of one call must share an element type");
+    }
+
+    a.cy0 = cy0; a.cy1 = cy1; a.cx0 = cx0; a.cx1 = cx1;
+    a.sgnv = a.sgnh = 1.f;
+    int rvk[6] = {0}, rhk[6] = {0};
+    for (int l = 0; l < 3; ++l) {
+        if (r[l] < 1 || r[l] > 16 || !lobes[l]) continue;
+        a.r[l] = r[l];
+        centre_index(lobes[l], r[l], l != 1, a.cv[l]);
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        rvk[k] = r[HV[k][1]]; rhk[k] = r[HV[k][0]];
+        if ((rc = set_out(a, k, outs[k], rvk[k], rhk[k], shift != 0))) return rc;
+    }
+    if ((rc = finish_args(a, *in, live, outs, rvk, rhk))) return rc;
+    a.mask = live;
+    a.rt = GAUSS_RADII[gauss_class(rmax)];
+    return TUCU_OK;
+}
+
+}   // namespace host
 }   // namespace tucu
 
 using namespace tucu;
+using namespace tucu::host;
 
 extern "C" {
 
@@ -332,63 +401,12 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
                          const tucu_image* in, const tucu_image* outs,
                          unsigned mask, int shift, void* stream)
 {
-    int rc;
-    if ((rc = check_image(in, "input", false))) return rc;
-    if (mask & ~unsigned(TUCU_ALL6)) return fail(TUCU_ERR_INVALID, "unknown output bits in mask");
-    if (mask == 0) return TUCU_OK;
-    if (!outs) return fail(TUCU_ERR_INVALID, "null output array");
-    const int r[3] = {n0 - 1, n1, n2 - 1};
     const float* lobes[3] = {lobe0, lobe1, lobe2};
-    static const int HV[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};   // (H lobe, V lobe)
-    const int nrow = in->nrow, ncol = in->ncol;
-
-    unsigned live = 0;
-    int rmax = 0, cy0 = nrow, cy1 = 0, cx0 = ncol, cx1 = 0;
-    for (int k = 0; k < 6; ++k) {
-        if (!(mask & (1u << k))) continue;
-        const int h = HV[k][0], v = HV[k][1];
-        for (int l : {h, v}) {
-            if (r[l] < 1 || r[l] > 16)
-                return fail(TUCU_ERR_UNSUPPORTED_LOBE, "lobe " + std::to_string(l) + " length outside the supported range");
-            if (!lobes[l]) return fail(TUCU_ERR_INVALID, "null lobe");
-        }
-        const int rv = r[v], rh = r[h];
-        if (nrow <= 2 * rv || ncol <= 2 * rh) continue;               // FIRFilter.h:304-312
-        const std::string name = "output " + std::to_string(k);
-        if ((rc = check_image(&outs[k], name.c_str(), true))) return rc;
-        if ((rc = check_extent(outs[k], rv, rh, nrow, ncol, shift != 0, name.c_str()))) return rc;
-        live |= 1u << k;
-        rmax = std::max(rmax, std::max(rv, rh));
-        cy0 = std::min(cy0, rv); cy1 = std::max(cy1, nrow - rv);
-        cx0 = std::min(cx0, rh); cx1 = std::max(cx1, ncol - rh);
-    }
-    if (!live) return TUCU_OK;
-    if (!in->data) return fail(TUCU_ERR_INVALID, "input: null data");
-
+    const int n[3] = {n0, n1, n2};
     SepArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.cy0 = cy0; a.cy1 = cy1; a.cx0 = cx0; a.cx1 = cx1;
-    a.mask = live;
-    a.sgnv = a.sgnh = 1.f;
-    int rvk[6] = {0}, rhk[6] = {0};
-    for (int l = 0; l < 3; ++l) {
-        if (r[l] < 1 || r[l] > 16 || !lobes[l]) continue;
-        a.r[l] = r[l];
-        centre_index(lobes[l], r[l], l != 1, a.cv[l]);
-    }
-    for (int k = 0; k < 6; ++k) {
-        if (!(live & (1u << k))) continue;
-        rvk[k] = r[HV[k][1]]; rhk[k] = r[HV[k][0]];
-        if ((rc = set_out(a, k, outs[k], rvk[k], rhk[k], shift != 0))) return rc;
-    }
-    if ((rc = finish_args(a, *in, live, outs, rvk, rhk))) return rc;
-    int odt = -1;
-    for (int k = 0; k < 6; ++k) {
-        if (!(live & (1u << k))) continue;
-        if (odt < 0) odt = outs[k].dtype;
-        else if (outs[k].dtype != odt)
-            return fail(TUCU_ERR_INVALID, "all outputs of one call must share an element type");
-    }
+    int rmax = 0, odt = 0;
+    int rc = gauss_args(lobes, n, in, outs, mask, shift, a, rmax, odt);
+    if (rc || !a.mask) return rc;
     const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
     if (!e->fn) e = &g_gauss_table[gauss_class(rmax)][GAUSS_TH_DEFAULT][in->dtype][odt];
     return launch(*e, a, static_cast<hipStream_t>(stream));

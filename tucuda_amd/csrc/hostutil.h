@@ -1,0 +1,23 @@
+// hostutil.h -- host-side helpers of the C ABI shared by its translation
+// units (defined in capi.hip).
+#pragma once
+
+#include "tucu.h"
+#include "sepfir.h"
+
+#include <hip/hip_runtime.h>
+#include <string>
+
+namespace tucu {
+namespace host {
+
+int    fail(int code, const std::string& msg);
+int    device_fail(hipError_t e, const char* what);
+size_t dtype_size(int dt);
+int    gauss_class(int r);
+int    check_image(const tucu_image* im, const char* name, bool need_data);
+int    gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in,
+                  const tucu_image* outs, unsigned mask, int shift, SepArgs& a, int& rmax, int& odt);
+
+}   // namespace host
+}   // namespace tucu

@@ -46,7 +46,21 @@ struct bf16_t { uint16_t bits; };
 using f16_t = _Float16;
 template <class T> using gptr = T __attribute__((address_space(1)))*;   // global memory
 typedef float    v4f32 __attribute__((ext_vector_type(4)));
+typedef float    v2f32 __attribute__((ext_vector_type(2)));
 typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+typedef _Float16 v2f16 __attribute__((ext_vector_type(2)));
+typedef __bf16   v2bf16 __attribute__((ext_vector_type(2)));
+
+// Two floats to two packed 16-bit values, round to nearest even
+// (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ uint32_t pack_f16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f32){a, b}, v2f16));
+}
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f32){a, b}, v2bf16));
+}
 
 struct SepArgs {
     const void* in;
@@ -63,10 +77,17 @@ struct SepArgs {
     float    cv[3][KMAX];          // vertical lobes (centre-indexed, zero padded); GAUSS: both passes
     float    ch[KMAX];             // FIR: horizontal lobe
     float    tv[TMAX], th[TMAX];   // TAPS: full vertical / horizontal taps, tap k at offset k - R
+    int      rt;                   // template radius class the tile grid was laid out for (batches)
     uintptr_t obase[6];            // output k: address of the element for centre (0, 0), i.e.
                                    // data + (dy_k * stride + dx_k) * elem (may lie before data)
     int32_t  ostride[6];           // output row stride, elements
 };
+
+// Job descriptors read by the batched kernel live in device memory the
+// kernel never writes; reading them through the constant address space lets
+// the compiler treat every field as invariant (scalar loads, hoisted out of
+// the tile loops), as it does for kernel arguments.
+typedef const __attribute__((address_space(4))) SepArgs ConstArgs;
 
 // ----------------------------------------------------------------- pixels
 template <class T> struct Px;
@@ -105,14 +126,10 @@ template <> struct Px<f16_t> {
     {
         return float(__builtin_bit_cast(f16_t, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0)));
     }
-    static __device__ __forceinline__ uint32_t bits(float v)
-    {
-        return uint32_t(__builtin_bit_cast(uint16_t, f16_t(v)));     // round to nearest even
-    }
     static __device__ __forceinline__ void st(gptr<f16_t> p, float v) { *p = f16_t(v); }
     static __device__ __forceinline__ void st4(gptr<f16_t> p, float a, float b, float c, float d)
     {
-        *reinterpret_cast<gptr<v2u32>>(p) = v2u32{bits(a) | bits(b) << 16, bits(c) | bits(d) << 16};
+        *reinterpret_cast<gptr<v2u32>>(p) = v2u32{pack_f16(a, b), pack_f16(c, d)};
     }
 };
 
@@ -121,26 +138,21 @@ template <> struct Px<bf16_t> {
     {
         return __uint_as_float(uint32_t(__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0)) << 16);
     }
-    // round to nearest even; NaN stays a quiet NaN
-    static __device__ __forceinline__ uint32_t bits(float v)
+    static __device__ __forceinline__ void st(gptr<bf16_t> p, float v)
     {
-        uint32_t x = __float_as_uint(v);
-        if ((x & 0x7fffffffu) > 0x7f800000u) return (x >> 16) | 0x40u;
-        x += 0x7fffu + ((x >> 16) & 1u);
-        return x >> 16;
+        p->bits = __builtin_bit_cast(uint16_t, __bf16(v));
     }
-    static __device__ __forceinline__ void st(gptr<bf16_t> p, float v) { p->bits = uint16_t(bits(v)); }
     static __device__ __forceinline__ void st4(gptr<bf16_t> p, float a, float b, float c, float d)
     {
-        *reinterpret_cast<gptr<v2u32>>(p) = v2u32{bits(a) | bits(b) << 16, bits(c) | bits(d) << 16};
+        *reinterpret_cast<gptr<v2u32>>(p) = v2u32{pack_bf16(a, b), pack_bf16(c, d)};
     }
 };
 
 // Results of centre columns xc .. xc+3 of centre row y for output k, whose
 // valid centre rectangle is rows [vy0, vy1) x columns [vx0, vx1).
 // fast: the caller has established that all four are valid and aligned.
-template <class OUT> __device__ __forceinline__ void
-store4(const SepArgs& a, int k, bool fast, int vy0, int vy1, int vx0, int vx1,
+template <class OUT, class A> __device__ __forceinline__ void
+store4(const A& a, int k, bool fast, int vy0, int vy1, int vx0, int vx1,
        int y, int xc, float v0, float v1, float v2, float v3)
 {
     const gptr<OUT> q = reinterpret_cast<gptr<OUT>>(a.obase[k] + uintptr_t(int64_t(y) * a.ostride[k] + xc) * sizeof(OUT));
@@ -161,8 +173,8 @@ store4(const SepArgs& a, int k, bool fast, int vy0, int vy1, int vx0, int vx1,
 
 // Symmetric lobe: sum_k c[k] * s[k] (k = R .. 1, outermost first, as the
 // reference orders its taps), then the centre tap.
-template <int R> __device__ __forceinline__ float
-sym_sum(const float* c, const float* s, float centre)
+template <int R, class P> __device__ __forceinline__ float
+sym_sum(P c, const float* s, float centre)
 {
     float acc = c[R] * s[R];
 #pragma unroll
@@ -171,8 +183,8 @@ sym_sum(const float* c, const float* s, float centre)
 }
 
 // Antisymmetric lobe: sum_k c[k] * d[k] with d[k] = (left or top) - (right or bottom).
-template <int R> __device__ __forceinline__ float
-anti_sum(const float* c, const float* d)
+template <int R, class P> __device__ __forceinline__ float
+anti_sum(P c, const float* d)
 {
     float acc = c[R] * d[R];
 #pragma unroll
@@ -191,8 +203,8 @@ __device__ __forceinline__ int xcd_remap(int b, int n)
 }
 
 // Vertical pass of one tile: NWIN input rows of column xi into w[].
-template <int NWIN, class IN> __device__ __forceinline__ void
-load_column(const SepArgs& a, __amdgpu_buffer_rsrc_t rsrc, int xi, int y_first, float* w)
+template <int NWIN, class IN, class A> __device__ __forceinline__ void
+load_column(const A& a, __amdgpu_buffer_rsrc_t rsrc, int xi, int y_first, float* w)
 {
     const bool colok = unsigned(xi) < unsigned(a.ncol);
     const int voff = (colok ? xi : 0) * int(sizeof(IN));
@@ -219,9 +231,9 @@ load_column(const SepArgs& a, __amdgpu_buffer_rsrc_t rsrc, int xi, int y_first, 
 // field 0 with the runtime parity sgnv (s[k] = w[y-k] + sgnv * w[y+k]).
 // Tap-outer loop: every row keeps its own accumulator, summed over
 // k = RV .. 1 then the centre, the order sym_sum uses.
-template <int RV, int TH, int PITCH, bool F0, bool F2, bool FIR>
+template <int RV, int TH, int PITCH, bool F0, bool F2, bool FIR, class A>
 __device__ __forceinline__ void
-vpass_sym(const SepArgs& a, const float* w, float* lds, int t)
+vpass_sym(const A& a, const float* w, float* lds, int t)
 {
     float acc0[TH], acc2[TH];
 #pragma unroll
@@ -242,9 +254,9 @@ vpass_sym(const SepArgs& a, const float* w, float* lds, int t)
 }
 
 // Vertical pass, antisymmetric field 1 (lobe cv[1]) from d[k] = w[y-k] - w[y+k].
-template <int RV, int TH, int PITCH>
+template <int RV, int TH, int PITCH, class A>
 __device__ __forceinline__ void
-vpass_anti(const SepArgs& a, const float* w, float* lds, int t)
+vpass_anti(const A& a, const float* w, float* lds, int t)
 {
     float acc[TH];
 #pragma unroll
@@ -261,9 +273,9 @@ vpass_anti(const SepArgs& a, const float* w, float* lds, int t)
 
 // Vertical pass with general taps (no symmetry): field 0 = sum_k tv[k] *
 // w[y - RV + k], k = 0 .. 2 RV, top tap first.
-template <int RV, int TH, int PITCH>
+template <int RV, int TH, int PITCH, class A>
 __device__ __forceinline__ void
-vpass_taps(const SepArgs& a, const float* w, float* lds, int t)
+vpass_taps(const A& a, const float* w, float* lds, int t)
 {
     float acc[TH];
 #pragma unroll
@@ -282,9 +294,9 @@ vpass_taps(const SepArgs& a, const float* w, float* lds, int t)
 //         field 2 -> diffVV (lobe 0)
 //   FIR   field 0 -> output 0 (lobe ch, runtime parity sgnh)
 // FAST: every store of the tile is valid and aligned (no checks).
-template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, bool FAST, class OUT, bool TAPS = false>
+template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, bool FAST, class OUT, bool TAPS, class A>
 __device__ __forceinline__ void
-hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
+hpass_field(const A& a, const float* lds, int t, int cy, int cx)
 {
     constexpr int NRUN = TWO / 4;
     constexpr int HW   = 2 * RH + 4;
@@ -326,7 +338,7 @@ hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
         } else {
         // Tap-outer: 4 columns x up to 3 lobes of running sums, each summed
         // over k = RH .. 1 then the centre (the order sym_sum uses).
-        const float* c0 = FIR ? a.ch : a.cv[0];
+        const auto c0 = FIR ? a.ch : a.cv[0];
 #pragma unroll
         for (int k = RH; k >= 1; --k) {
 #pragma unroll
@@ -356,31 +368,33 @@ hpass_field(const SepArgs& a, const float* lds, int t, int cy, int cx)
     }
 }
 
-template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, class OUT, bool TAPS = false>
+template <int RH, int TH, int PITCH, int TWO, int F, bool FIR, class OUT, bool TAPS = false, class A>
 __device__ __forceinline__ void
-hpass(const SepArgs& a, const float* lds, int t, int cy, int cx, bool fast)
+hpass(const A& a, const float* lds, int t, int cy, int cx, bool fast)
 {
     if (fast) hpass_field<RH, TH, PITCH, TWO, F, FIR, true, OUT, TAPS>(a, lds, t, cy, cx);
     else      hpass_field<RH, TH, PITCH, TWO, F, FIR, false, OUT, TAPS>(a, lds, t, cy, cx);
 }
 
-template <int RV, int RH, int TH, int MODE, class IN, class OUT>
-__global__ void __launch_bounds__(NT)
-sepfir_kernel(const SepArgs a)
+// Shared-memory floats one tile needs.
+template <int TH, int MODE>
+constexpr int tile_lds_floats() { return (MODE == MODE_GAUSS ? 3 : 1) * TH * (NT + 4); }
+
+// One tile (number `tile` of the image described by `a`, strip-major) of the
+// fused separable filter.  `lds` holds tile_lds_floats<TH, MODE>() floats,
+// 16-byte aligned.  Every thread of the workgroup calls it.
+template <int RV, int RH, int TH, int MODE, class IN, class OUT, class A>
+__device__ __forceinline__ void
+sepfir_tile(const A& a, int tile, float* lds)
 {
     constexpr bool TAPS = MODE == MODE_TAPS;
     constexpr bool FIR  = MODE == MODE_FIR || TAPS;     // one field, one output
-    constexpr int NF    = FIR ? 1 : 3;
     constexpr int TWO   = NT - 2 * RH;          // centre columns per tile
     constexpr int PITCH = NT + 4;               // LDS row pitch (floats), keeps 16-byte rows
     constexpr int NWIN  = TH + 2 * RV;          // input rows per column
     static_assert(TWO % 4 == 0 && (2 * RH + 4) % 4 == 0, "tile geometry");
     static_assert(RV <= 16 && RH <= 16, "radius");
 
-    __shared__ float4 lds4[NF * TH * PITCH / 4];
-    float* const lds = reinterpret_cast<float*>(lds4);
-
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = tile / a.tiles_y;
     const int ty = tile - tx * a.tiles_y;
     const int cx = a.cx0 + tx * TWO;
@@ -422,6 +436,15 @@ sepfir_kernel(const SepArgs a)
         if (needF1) hpass<RH, TH, PITCH, TWO, 1, false, OUT>(a, lds, t, cy, cx, fast);
         if (needF2) hpass<RH, TH, PITCH, TWO, 2, false, OUT>(a, lds, t, cy, cx, fast);
     }
+}
+
+// One image: one workgroup per tile.
+template <int RV, int RH, int TH, int MODE, class IN, class OUT>
+__global__ void __launch_bounds__(NT)
+sepfir_kernel(const SepArgs a)
+{
+    __shared__ float4 lds4[tile_lds_floats<TH, MODE>() / 4];
+    sepfir_tile<RV, RH, TH, MODE, IN, OUT>(a, xcd_remap(blockIdx.x, gridDim.x), reinterpret_cast<float*>(lds4));
 }
 
 // Host-side launch entry of one instantiation.

@@ -231,3 +231,84 @@ class FIRGaussianConvolver2(FIRFilter2):
         check(_capi.load().tucu_gauss2_convolve(
             _fptr(l0), l0.size, _fptr(l1), l1.size, _fptr(l2), l2.size,
             ctypes.byref(src), descs, mask, int(bool(shift)), _stream_handle(inp)))
+
+
+class GaussBatch:
+    """Batched mode (extension): many Gaussian jobs of uneven sizes and
+    sigmas in one persistent launch.  ``jobs`` is a sequence of
+    ``(input, outputs, lobes_or_sigma)`` or ``(input, outputs, lobes_or_sigma,
+    shift)``; ``outputs`` maps method names to tensors (or lists six
+    tensors / None); a float sigma takes lobes by ``rule``.  Each job gets
+    exactly what FIRGaussianConvolver2.convolve6 would write for it.  The
+    tensors must outlive the batch."""
+
+    def __init__(self, jobs, rule: int = _capi.TUCU_RULE_REFERENCE):
+        arr = (_capi.TucuGaussJob * max(1, len(jobs)))()
+        keep = []
+        self._device = None
+        for i, job in enumerate(jobs):
+            inp, outs, lob = job[0], job[1], job[2]
+            shift = job[3] if len(job) > 3 else True
+            if isinstance(lob, (int, float)):
+                lob = gauss_lobes(float(lob), rule)
+            lob = [_f32(x) for x in lob]
+            keep.append(lob)
+            if isinstance(outs, Mapping):
+                seq = [outs.get(m) for m in METHOD_ORDER]
+            else:
+                seq = list(outs) + [None] * (6 - len(outs))
+            j = arr[i]
+            j.in_ = image_desc(inp)
+            mask = 0
+            for k, t in enumerate(seq[:6]):
+                if t is not None:
+                    j.out[k] = image_desc(t)
+                    mask |= 1 << k
+            j.lobe0, j.lobe1, j.lobe2 = (_fptr(x) for x in lob)
+            j.n0, j.n1, j.n2 = (x.size for x in lob)
+            j.mask, j.shift = mask, int(bool(shift))
+            self._device = inp.device
+        h = ctypes.c_void_p()
+        check(_capi.load().tucu_batch_create(ctypes.byref(h), arr, len(jobs)))
+        self._h = h
+
+    def _stream(self):
+        if self._device is None:
+            return ctypes.c_void_p(None)
+        return ctypes.c_void_p(torch.cuda.current_stream(self._device).cuda_stream)
+
+    def launch(self) -> None:
+        check(_capi.load().tucu_batch_launch(self._h, self._stream()))
+
+    def set_tile_rows(self, rows: int) -> None:
+        check(_capi.load().tucu_batch_set_tile_rows(self._h, int(rows)))
+
+    def autotune(self, reps: int = 5):
+        rows, us = ctypes.c_int(0), ctypes.c_float(0.0)
+        check(_capi.load().tucu_batch_autotune(self._h, self._stream(), int(reps),
+                                               ctypes.byref(rows), ctypes.byref(us)))
+        return rows.value, us.value
+
+    def info(self) -> dict:
+        v = [ctypes.c_int(0) for _ in range(4)]
+        check(_capi.load().tucu_batch_info(self._h, *(ctypes.byref(x) for x in v)))
+        return dict(zip(("jobs", "tiles", "grid", "tile_rows"), (x.value for x in v)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _capi.load().tucu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_cache_save(path: str) -> None:
+    check(_capi.load().tucu_tune_cache_save(path.encode()))
+
+
+def tune_cache_load(path: str) -> None:
+    check(_capi.load().tucu_tune_cache_load(path.encode()))
