@@ -67,6 +67,35 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0):
             "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)"}
 
 
+def served_leg(dev, rates: str, seconds: float) -> dict:
+    """p50/p99 latency of the continuous-batching server under Poisson
+    arrivals: 256 resident bfloat16 images (20% 3840x2160, the rest
+    log-uniform in [240, 2160]), random sigma in [0.8, 4] and output subset."""
+    import torch
+    from tucuda_amd import serve_bench
+    from tucuda_amd.serve import Server
+    images = serve_bench.setup(256, 5, dev)
+    torch.cuda.synchronize()
+    out = {"workload": "Poisson arrivals; 256 resident bf16 images, 20% 3840x2160, rest log-uniform "
+                       "[240,2160]; sigma U[0.8,4], radius ceil(4 sigma); random output subset; bf16 out",
+           "seconds_per_rate": seconds, "latency": "completion - scheduled arrival", "rates": []}
+    with Server(16 << 30) as srv:
+        serve_bench.run_stream(srv, images, 500.0, 0.5, 104, check_every=0)
+        for k, rate in enumerate(float(r) for r in rates.split(",") if r):
+            res = serve_bench.run_stream(srv, images, rate, seconds, 110 + k)
+            log(f"served {rate:.0f} req/s: p50 {res['p50_ms']} ms p99 {res['p99_ms']} ms, "
+                f"mean batch {res['mean_batch']}, {res['mismatched']} of {res['checked']} checks differ")
+            out["rates"].append({k2: res[k2] for k2 in (
+                "rate_req_s", "requests", "served_req_s", "served_gpix_s", "p50_ms", "p99_ms", "max_ms",
+                "mean_batch", "checked", "mismatched")})
+        _, _, cap = serve_bench.run_capacity(srv, images, 2000, 177)
+        out["capacity"] = cap
+        log(f"served capacity: {cap['served_gpix_s']} Gpix/s")
+    del images
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +109,9 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch every step from Python instead of replaying a captured HIP graph")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="seconds of Poisson arrivals per served-stream rate (0 skips the served leg)")
+    ap.add_argument("--serve-rates", default="200,1000,4000", help="served-stream arrival rates, req/s")
     ap.add_argument("--nrow", type=int, default=NROW, help="image rows (the headline workload is 1080)")
     ap.add_argument("--ncol", type=int, default=NCOL, help="image columns (the headline workload is 1920)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -243,6 +275,8 @@ def main():
             "algorithmic_bytes": alg_bytes,
         },
     }
+    if args.serve_seconds > 0:
+        line["served"] = served_leg(dev, args.serve_rates, args.serve_seconds)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline(img, SIGMA, args.cpu_budget)

@@ -202,6 +202,64 @@ TUCU_API int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* gri
 TUCU_API int tucu_tune_cache_save(const char* path);
 TUCU_API int tucu_tune_cache_load(const char* path);
 
+/* ---- serving (extension) -------------------------------------------------
+ * A continuous-batching server for Gaussian requests, no counterpart in the
+ * reference.  Requests are submitted from any thread; a scheduler thread
+ * admits them in arrival order while the output pool (a fixed HBM budget,
+ * pool_bytes) has room, packs all admitted requests into one batched launch
+ * on a free in-flight slot (a stream each, max_inflight of them), so requests
+ * that arrive while the GPU works join the next launch; completions are
+ * drained with tucu_server_poll and their outputs, which live in the pool,
+ * are handed back with tucu_server_release.  Each output equals what
+ * tucu_gauss2_convolve writes for the request alone. */
+
+typedef struct tucu_server_s* tucu_server;
+
+typedef struct {
+    size_t  pool_bytes;       /* HBM budget for request outputs (admission control) */
+    int32_t in_dtype;         /* element type of every request's input */
+    int32_t out_dtype;        /* element type of the outputs */
+    int32_t rule;             /* TUCU_RULE_* for the lobes of each sigma */
+    int32_t tile_rows;        /* batched-kernel tile height: 4, 8 or 16 (0: 8) */
+    int32_t max_batch_jobs;   /* requests per launch (0: 1024) */
+    int32_t max_inflight;     /* launches in flight at once (0: 2) */
+    int32_t max_tiles;        /* tiles per launch (0: 2^21) */
+    int32_t reserved;
+} tucu_server_config;
+
+typedef struct {
+    tucu_image in;            /* input in device memory, valid until the request completes */
+    float      sigma;
+    uint32_t   mask;          /* outputs wanted (TUCU_SMOOTH ...) */
+    int32_t    shift;
+    int32_t    reserved;
+    double     arrival;       /* arrival time on the server clock, reported back */
+} tucu_request;
+
+typedef struct {
+    uint64_t   id;
+    int32_t    status;        /* tucu_status of the request */
+    int32_t    reserved;
+    double     arrival, admitted, launched, completed;   /* server clock, seconds */
+    tucu_image out[6];        /* outputs (pool memory), valid until tucu_server_release(id) */
+} tucu_completion;
+
+typedef struct {
+    uint64_t batches, requests_launched, requests_done, max_batch;
+    uint64_t pool_bytes, pool_used, pool_peak;
+} tucu_server_counters;
+
+TUCU_API int    tucu_server_create(tucu_server* server, const tucu_server_config* cfg);
+/* Fails requests not yet admitted, waits for in-flight launches, frees everything. */
+TUCU_API int    tucu_server_destroy(tucu_server server);
+/* Seconds since the server was created (the clock of all timestamps). */
+TUCU_API double tucu_server_now(tucu_server server);
+TUCU_API int    tucu_server_submit(tucu_server server, const tucu_request* req, uint64_t* id);
+/* Moves up to max completions into out; *n = number moved. */
+TUCU_API int    tucu_server_poll(tucu_server server, tucu_completion* out, int max, int* n);
+TUCU_API int    tucu_server_release(tucu_server server, uint64_t id);
+TUCU_API int    tucu_server_stats(tucu_server server, tucu_server_counters* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,3 +71,18 @@ def random_lobe(rng: np.random.Generator, n: int) -> np.ndarray:
     lobe = rng.standard_normal(n)
     full = np.abs(oracle.full_kernel(lobe)).sum()
     return (lobe / full).astype(np.float32)
+
+
+def full_taps(lobe, symmetric: bool) -> np.ndarray:
+    """The 2r+1 taps of a lobe (outermost first; symmetric lobes end with the
+    centre tap, antisymmetric ones weight the left side + and the right -)."""
+    lobe = np.asarray(lobe, np.float64)
+    r = lobe.size - 1 if symmetric else lobe.size
+    k = np.zeros(2 * r + 1)
+    for i in range(1, r + 1):
+        c = lobe[r - i]
+        k[r - i] = c
+        k[r + i] = c if symmetric else -c
+    if symmetric:
+        k[r] = lobe[r]
+    return k

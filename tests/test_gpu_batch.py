@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._util import assert_f32_parity, to_host
+from tests._util import assert_f32_parity, full_taps, to_host
 from tucuda_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -44,16 +44,7 @@ def _single(dev, job):
 
 
 def _full_taps(lobe, symmetric):
-    lobe = np.asarray(lobe, np.float64)
-    r = lobe.size - 1 if symmetric else lobe.size
-    k = np.zeros(2 * r + 1)
-    for i in range(1, r + 1):
-        c = lobe[r - i]
-        k[r - i] = c
-        k[r + i] = c if symmetric else -c
-    if symmetric:
-        k[r] = lobe[r]
-    return k.astype(np.float32)
+    return full_taps(lobe, symmetric).astype(np.float32)
 
 
 @pytest.mark.parametrize("rule", [0, 1])

@@ -30,6 +30,8 @@ EXPORTS = (
     "tucu_lobe_size", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
     "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
+    "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
+    "tucu_server_poll", "tucu_server_release", "tucu_server_stats",
 )
 
 
@@ -57,6 +59,50 @@ class TucuGaussJob(ctypes.Structure):
         ("mask", ctypes.c_uint32),
         ("shift", ctypes.c_int32),
     ]
+
+
+class TucuServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_size_t),
+        ("in_dtype", ctypes.c_int32),
+        ("out_dtype", ctypes.c_int32),
+        ("rule", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("max_batch_jobs", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("max_tiles", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class TucuRequest(ctypes.Structure):
+    _fields_ = [
+        ("in_", TucuImage),
+        ("sigma", ctypes.c_float),
+        ("mask", ctypes.c_uint32),
+        ("shift", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("arrival", ctypes.c_double),
+    ]
+
+
+class TucuCompletion(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("arrival", ctypes.c_double),
+        ("admitted", ctypes.c_double),
+        ("launched", ctypes.c_double),
+        ("completed", ctypes.c_double),
+        ("out", TucuImage * 6),
+    ]
+
+
+class TucuServerCounters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "batches", "requests_launched", "requests_done", "max_batch",
+        "pool_bytes", "pool_used", "pool_peak")]
 
 
 class TucuError(RuntimeError):
@@ -133,8 +179,16 @@ def load() -> ctypes.CDLL:
     L.tucu_batch_info.argtypes = [vp, ip, ip, ip, ip]
     L.tucu_tune_cache_save.argtypes = [ctypes.c_char_p]
     L.tucu_tune_cache_load.argtypes = [ctypes.c_char_p]
+    L.tucu_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TucuServerConfig)]
+    L.tucu_server_destroy.argtypes = [vp]
+    L.tucu_server_now.argtypes = [vp]
+    L.tucu_server_now.restype = ctypes.c_double
+    L.tucu_server_submit.argtypes = [vp, ctypes.POINTER(TucuRequest), ctypes.POINTER(ctypes.c_uint64)]
+    L.tucu_server_poll.argtypes = [vp, ctypes.POINTER(TucuCompletion), ctypes.c_int, ip]
+    L.tucu_server_release.argtypes = [vp, ctypes.c_uint64]
+    L.tucu_server_stats.argtypes = [vp, ctypes.POINTER(TucuServerCounters)]
     for name in EXPORTS:
-        if name != "tucu_last_error" and name != "tucu_row_stride" and name != "tucu_version":
+        if name not in ("tucu_last_error", "tucu_row_stride", "tucu_version", "tucu_server_now"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -20,6 +20,42 @@ namespace tucu {
 
 BatchEntry g_batch_table[NBTH][4][4];
 
+namespace host {
+
+// Tile list of `jobs` for tile height th (sets every job's tiles_y): tiles of
+// the larger radius classes first (they cost most; handing them out early
+// shortens the tail), each job's tiles in strip-major order.
+int make_tiles(std::vector<SepArgs>& jobs, int th, std::vector<BatchTile>& tiles)
+{
+    for (SepArgs& a : jobs) a.tiles_y = (a.cy1 - a.cy0 + th - 1) / th;
+    std::vector<int> order(jobs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].rt > jobs[y].rt; });
+    tiles.clear();
+    for (int j : order) {
+        const SepArgs& a = jobs[j];
+        const int two = NT - 2 * a.rt;
+        const int64_t n = int64_t((a.cx1 - a.cx0 + two - 1) / two) * a.tiles_y;
+        for (int64_t t = 0; t < n; ++t) tiles.push_back({uint32_t(j), uint32_t(t)});
+    }
+    if (tiles.size() >= size_t(INT32_MAX)) return fail(TUCU_ERR_INVALID, "batch too large");
+    return TUCU_OK;
+}
+
+// Workgroups of a batched launch: enough to fill the GPU once, at most one
+// per tile.
+int batch_grid(const BatchEntry& e, int ntiles)
+{
+    int dev = 0, ncu = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, NT, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return std::max(1, std::min(ntiles, std::max(1, ncu) * per_cu));
+}
+
+}   // namespace host
+
 namespace {
 
 using namespace host;
@@ -67,28 +103,16 @@ int free_layout(Layout& l)
     return TUCU_OK;
 }
 
-// Tile grid of every job for tile height th, tiles of the larger radius
-// classes first (they cost most; handing them out early shortens the tail).
 int build_layout(tucu_batch_s& b, int thi)
 {
     Layout& l = b.layout[thi];
     if (l.built) return TUCU_OK;
     const BatchEntry& e = g_batch_table[thi][b.in_dt][b.out_dt];
     if (!e.fn) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
-    const int th = e.th;
     std::vector<SepArgs> jobs = b.jobs;
-    for (SepArgs& a : jobs) a.tiles_y = (a.cy1 - a.cy0 + th - 1) / th;
-    std::vector<int> order(jobs.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].rt > jobs[y].rt; });
     std::vector<BatchTile> all;
-    for (int j : order) {
-        const SepArgs& a = jobs[j];
-        const int two = NT - 2 * a.rt;
-        const int64_t n = int64_t((a.cx1 - a.cx0 + two - 1) / two) * a.tiles_y;
-        for (int64_t t = 0; t < n; ++t) all.push_back({uint32_t(j), uint32_t(t)});
-    }
-    if (all.size() >= size_t(INT32_MAX)) return fail(TUCU_ERR_INVALID, "batch too large");
+    int rc = make_tiles(jobs, e.th, all);
+    if (rc) return rc;
     hipError_t err;
     if ((err = hipMalloc(&l.d_jobs, jobs.size() * sizeof(SepArgs))) != hipSuccess ||
         (err = hipMalloc(&l.d_tiles, std::max<size_t>(1, all.size()) * sizeof(BatchTile))) != hipSuccess) {
@@ -102,13 +126,8 @@ int build_layout(tucu_batch_s& b, int thi)
         free_layout(l);
         return device_fail(err, "hipMemcpy");
     }
-    int dev = 0, ncu = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, NT, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
     l.ntiles = int(all.size());
-    l.grid = std::max(1, std::min(l.ntiles, std::max(1, ncu) * per_cu));
+    l.grid = batch_grid(e, l.ntiles);
     l.built = true;
     return TUCU_OK;
 }

@@ -1,12 +1,14 @@
 // hostutil.h -- host-side helpers of the C ABI shared by its translation
-// units (defined in capi.hip).
+// units (defined in capi.hip and batch_host.hip).
 #pragma once
 
 #include "tucu.h"
 #include "sepfir.h"
+#include "batch.h"
 
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 
 namespace tucu {
 namespace host {
@@ -18,6 +20,8 @@ int    gauss_class(int r);
 int    check_image(const tucu_image* im, const char* name, bool need_data);
 int    gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in,
                   const tucu_image* outs, unsigned mask, int shift, SepArgs& a, int& rmax, int& odt);
+int    make_tiles(std::vector<SepArgs>& jobs, int th, std::vector<BatchTile>& tiles);
+int    batch_grid(const BatchEntry& e, int ntiles);
 
 }   // namespace host
 }   // namespace tucu
