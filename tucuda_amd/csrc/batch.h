@@ -11,6 +11,8 @@
 
 #include "sepfir.h"
 
+#include <type_traits>
+
 namespace tucu {
 
 struct BatchTile {
@@ -66,9 +68,10 @@ using BatchLaunchFn = hipError_t (*)(const SepArgs*, const BatchTile*, int ntile
                                      int grid, hipStream_t);
 
 struct BatchEntry {
-    BatchLaunchFn fn;
-    const void*   kernel;     // for occupancy queries
-    int           th;
+    BatchLaunchFn fn = nullptr;
+    const void*   kernel = nullptr;   // for occupancy queries
+    int           th = 0;             // tile rows
+    int           tw = 0;             // tile columns (0: NT - 2 x radius class)
 };
 
 template <int TH, class IN, class OUT>
@@ -87,5 +90,15 @@ constexpr int BATCH_TH_DEFAULT = 1;
 // [tile-height index][in dtype][out dtype].  Every height for the same-type
 // f32 and bf16 pairs, the default height for the other pairs.
 extern BatchEntry g_batch_table[NBTH][4][4];
+
+// Matrix-core batched kernels, [workgroup-shape index][in dtype][out dtype]
+// (tile rows 16, 32, 64), filled by the mfma_<type> units.
+extern BatchEntry g_batch_mfma_table[NBTH][4][4];
+
+template <class T> constexpr int dtype_index()
+{
+    return std::is_same_v<T, uint8_t> ? 0 : std::is_same_v<T, float> ? 1 : std::is_same_v<T, f16_t> ? 2 : 3;
+}
+
 
 }   // namespace tucu

@@ -19,14 +19,21 @@
 namespace tucu {
 
 BatchEntry g_batch_table[NBTH][4][4];
+BatchEntry g_batch_mfma_table[NBTH][4][4];
 
 namespace host {
 
-// Tile list of `jobs` for tile height th (sets every job's tiles_y): tiles of
-// the larger radius classes first (they cost most; handing them out early
-// shortens the tail), each job's tiles in strip-major order.
-int make_tiles(std::vector<SepArgs>& jobs, int th, std::vector<BatchTile>& tiles)
+const BatchEntry& batch_entry(int thi, int in_dt, int out_dt)
 {
+    return gauss_use_mfma() ? g_batch_mfma_table[thi][in_dt][out_dt] : g_batch_table[thi][in_dt][out_dt];
+}
+
+// Tile list of `jobs` for the tile shape of e (sets every job's tiles_y):
+// tiles of the larger radius classes first (they cost most; handing them out
+// early shortens the tail), each job's tiles in strip-major order.
+int make_tiles(std::vector<SepArgs>& jobs, const BatchEntry& e, std::vector<BatchTile>& tiles)
+{
+    const int th = e.th;
     for (SepArgs& a : jobs) a.tiles_y = (a.cy1 - a.cy0 + th - 1) / th;
     std::vector<int> order(jobs.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
@@ -34,7 +41,7 @@ int make_tiles(std::vector<SepArgs>& jobs, int th, std::vector<BatchTile>& tiles
     tiles.clear();
     for (int j : order) {
         const SepArgs& a = jobs[j];
-        const int two = NT - 2 * a.rt;
+        const int two = e.tw ? e.tw : NT - 2 * a.rt;
         const int64_t n = int64_t((a.cx1 - a.cx0 + two - 1) / two) * a.tiles_y;
         for (int64_t t = 0; t < n; ++t) tiles.push_back({uint32_t(j), uint32_t(t)});
     }
@@ -107,11 +114,11 @@ int build_layout(tucu_batch_s& b, int thi)
 {
     Layout& l = b.layout[thi];
     if (l.built) return TUCU_OK;
-    const BatchEntry& e = g_batch_table[thi][b.in_dt][b.out_dt];
+    const BatchEntry& e = batch_entry(thi, b.in_dt, b.out_dt);
     if (!e.fn) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
     std::vector<SepArgs> jobs = b.jobs;
     std::vector<BatchTile> all;
-    int rc = make_tiles(jobs, e.th, all);
+    int rc = make_tiles(jobs, e, all);
     if (rc) return rc;
     hipError_t err;
     if ((err = hipMalloc(&l.d_jobs, jobs.size() * sizeof(SepArgs))) != hipSuccess ||
@@ -146,8 +153,8 @@ int launch_layout(tucu_batch_s& b, int thi, hipStream_t s)
     if ((rc = build_layout(b, thi))) return rc;
     const Layout& l = b.layout[thi];
     if (l.ntiles == 0) return TUCU_OK;
-    const hipError_t e = g_batch_table[thi][b.in_dt][b.out_dt].fn(l.d_jobs, l.d_tiles, l.ntiles,
-                                                                   b.d_counter, l.grid, s);
+    const hipError_t e = batch_entry(thi, b.in_dt, b.out_dt).fn(l.d_jobs, l.d_tiles, l.ntiles,
+                                                                 b.d_counter, l.grid, s);
     return e == hipSuccess ? TUCU_OK : device_fail(e, "batched kernel launch");
 }
 
@@ -215,7 +222,8 @@ int tucu_batch_set_tile_rows(tucu_batch batch, int rows)
         thi = pick_th(*batch);
     } else {
         for (int i = 0; i < NBTH; ++i)
-            if (BATCH_THS[i] == rows && g_batch_table[i][batch->in_dt][batch->out_dt].fn) thi = i;
+            if (batch_entry(i, batch->in_dt, batch->out_dt).th == rows &&
+                batch_entry(i, batch->in_dt, batch->out_dt).fn) thi = i;
         batch->th_index = thi;
     }
     if (thi >= 0) {
@@ -242,7 +250,7 @@ int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* t
     if (njobs) *njobs = int(batch->jobs.size());
     if (ntiles) *ntiles = l.ntiles;
     if (grid) *grid = l.grid;
-    if (tile_rows) *tile_rows = BATCH_THS[thi];
+    if (tile_rows) *tile_rows = batch_entry(thi, batch->in_dt, batch->out_dt).th;
     return TUCU_OK;
 }
 
@@ -259,7 +267,7 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
     float best_t = 0.f;
     int rc = TUCU_OK;
     for (int i = 0; i < NBTH && rc == TUCU_OK; ++i) {
-        if (!g_batch_table[i][batch->in_dt][batch->out_dt].fn) continue;
+        if (!batch_entry(i, batch->in_dt, batch->out_dt).fn) continue;
         if ((rc = launch_layout(*batch, i, s))) break;                  // builds and warms up
         std::vector<float> t(reps);
         for (int r = 0; r < reps && rc == TUCU_OK; ++r) {
@@ -283,7 +291,7 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune[TuneKey(batch->in_dt, batch->out_dt, batch->rclass, batch->bucket)] = {best, best_t};
     }
-    if (best_rows) *best_rows = BATCH_THS[best];
+    if (best_rows) *best_rows = batch_entry(best, batch->in_dt, batch->out_dt).th;
     if (best_us) *best_us = best_t;
     return TUCU_OK;
 }
@@ -297,7 +305,8 @@ int tucu_tune_cache_save(const char* path)
     for (const auto& kv : g_tune)
         f << "{\"in\": " << std::get<0>(kv.first) << ", \"out\": " << std::get<1>(kv.first)
           << ", \"radius\": " << std::get<2>(kv.first) << ", \"log2_pixels\": " << std::get<3>(kv.first)
-          << ", \"tile_rows\": " << BATCH_THS[kv.second.th_index] << ", \"us\": " << kv.second.us << "}\n";
+          << ", \"tile_rows\": " << batch_entry(kv.second.th_index, std::get<0>(kv.first), std::get<1>(kv.first)).th
+          << ", \"us\": " << kv.second.us << "}\n";
     return TUCU_OK;
 }
 
@@ -315,7 +324,7 @@ int tucu_tune_cache_load(const char* path)
                                       "\"tile_rows\": %d, \"us\": %f}", &in, &out, &r, &bucket, &rows, &us) != 6)
             continue;
         for (int i = 0; i < NBTH; ++i)
-            if (BATCH_THS[i] == rows && in >= 0 && in < 4 && out >= 0 && out < 4 && g_batch_table[i][in][out].fn)
+            if (in >= 0 && in < 4 && out >= 0 && out < 4 && batch_entry(i, in, out).fn && batch_entry(i, in, out).th == rows)
                 g_tune[TuneKey(in, out, r, bucket)] = {i, us};
     }
     return TUCU_OK;

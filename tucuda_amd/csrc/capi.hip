@@ -8,10 +8,12 @@
 // TUCU_ERR_DEVICE and leaves the decision to the caller.
 #include "tucu.h"
 #include "sepfir.h"
+#include "mfma.h"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -21,6 +23,7 @@ namespace tucu {
 KernelEntry g_fir_table[4][4][4][4];
 KernelEntry g_taps_table[4][4][4][4];
 KernelEntry g_gauss_table[5][NGTH][4][4];
+KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
@@ -71,6 +74,31 @@ int gauss_class(int r)
 
 // Tile-height choice of the fused Gaussian kernel: TUCU_GAUSS_TH (one of
 // GAUSS_THS) overrides the default.
+// Six-output Gaussian kernel family: matrix cores (default) or the direct
+// FP32 vector kernel (TUCU_GAUSS_PATH=valu).
+bool gauss_use_mfma()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("TUCU_GAUSS_PATH");
+        return !(e && std::strcmp(e, "valu") == 0);
+    }();
+    return on;
+}
+
+// Workgroup shape of the single-image matrix-core kernel: TUCU_MFMA_ROWS
+// (16, 32 or 64 tile rows) overrides the default (32).
+int mfma_wy_index()
+{
+    static const int idx = [] {
+        const char* e = std::getenv("TUCU_MFMA_ROWS");
+        if (e)
+            for (int i = 0; i < NMWY; ++i)
+                if (std::atoi(e) == 16 * MFMA_WYS[i]) return i;
+        return MFMA_WY_DEFAULT;
+    }();
+    return idx;
+}
+
 int gauss_th_index()
 {
     static const int idx = [] {
@@ -101,6 +129,36 @@ void centre_index(const float* lobe, int r, bool symmetric, float c[KMAX])
     std::fill(c, c + KMAX, 0.f);
     for (int k = 1; k <= r; ++k) c[k] = lobe[r - k];
     if (symmetric) c[0] = lobe[r];
+}
+
+// Power-of-two operand scales of the matrix-core kernels (mfma.h): taps of
+// lobe l times mp[l] peak in [2^11, 2^12); mq makes mq * mp[l] * (L1 norm of
+// lobe l) <= 1 for every lobe, so that fields of an input scaled below 2^13
+// stay below 2^13; mu[k] undoes the tap and input scales of output k = (H
+// lobe, V lobe).
+void mfma_scales(SepArgs& a)
+{
+    static const int HV[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};
+    int ep[3], emax = 0;
+    for (int l = 0; l < 3; ++l) {
+        float cmax = 0.f, l1 = 0.f;
+        for (int k = 0; k < KMAX; ++k) {
+            const float c = std::fabs(a.cv[l][k]);
+            cmax = std::max(cmax, c);
+            l1 += k == 0 ? c : 2.f * c;
+        }
+        int e = 0;
+        (void)std::frexp(cmax, &e);             // cmax < 2^e
+        ep[l] = 12 - e;
+        a.mp[l] = std::ldexp(1.f, ep[l]);
+        (void)std::frexp(l1 * a.mp[l], &e);     // l1 * mp < 2^e
+        emax = std::max(emax, e);
+    }
+    a.mq = std::ldexp(1.f, -emax);
+    for (int k = 0; k < 6; ++k) {
+        const int h = HV[k][0], v = HV[k][1];
+        a.mu[k] = std::ldexp(1.f, -(ep[h] + ep[v] - emax));
+    }
 }
 
 // Output k of the kernel: base address of centre (0, 0) and row stride.
@@ -155,7 +213,7 @@ int check_extent(const tucu_image& o, int rv, int rh, int nrow, int ncol, bool s
 int launch(const KernelEntry& k, SepArgs& a, hipStream_t s)
 {
     if (!k.fn) return fail(TUCU_ERR_INVALID, "no kernel for this type / radius combination");
-    const int two = NT - 2 * k.rh;
+    const int two = k.tw ? k.tw : NT - 2 * k.rh;
     const int tiles_x = (a.cx1 - a.cx0 + two - 1) / two;
     const int tiles_y = (a.cy1 - a.cy0 + k.th - 1) / k.th;
     a.tiles_y = tiles_y;
@@ -230,6 +288,7 @@ int gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in
         if ((rc = set_out(a, k, outs[k], rvk[k], rhk[k], shift != 0))) return rc;
     }
     if ((rc = finish_args(a, *in, live, outs, rvk, rhk))) return rc;
+    mfma_scales(a);
     a.mask = live;
     a.rt = GAUSS_RADII[gauss_class(rmax)];
     return TUCU_OK;
@@ -407,6 +466,12 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
     int rmax = 0, odt = 0;
     int rc = gauss_args(lobes, n, in, outs, mask, shift, a, rmax, odt);
     if (rc || !a.mask) return rc;
+    if (gauss_use_mfma()) {
+        const int rc = rmax <= 8 ? 0 : 1;
+        const KernelEntry* m = &g_gauss_mfma_table[rc][mfma_wy_index()][in->dtype][odt];
+        if (!m->fn) m = &g_gauss_mfma_table[rc][MFMA_WY_DEFAULT][in->dtype][odt];
+        return launch(*m, a, static_cast<hipStream_t>(stream));
+    }
     const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
     if (!e->fn) e = &g_gauss_table[gauss_class(rmax)][GAUSS_TH_DEFAULT][in->dtype][odt];
     return launch(*e, a, static_cast<hipStream_t>(stream));

@@ -20,7 +20,11 @@ int    gauss_class(int r);
 int    check_image(const tucu_image* im, const char* name, bool need_data);
 int    gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in,
                   const tucu_image* outs, unsigned mask, int shift, SepArgs& a, int& rmax, int& odt);
-int    make_tiles(std::vector<SepArgs>& jobs, int th, std::vector<BatchTile>& tiles);
+bool   gauss_use_mfma();
+// Batched kernel of tile-shape index thi (0 .. NBTH-1) for a type pair, from
+// the kernel family in use (matrix-core by default).
+const BatchEntry& batch_entry(int thi, int in_dt, int out_dt);
+int    make_tiles(std::vector<SepArgs>& jobs, const BatchEntry& e, std::vector<BatchTile>& tiles);
 int    batch_grid(const BatchEntry& e, int ntiles);
 
 }   // namespace host

@@ -1,0 +1,477 @@
+// mfma.h -- matrix-core (MFMA) variant of the fused six-output Gaussian filter.
+//
+// The direct FP32 filter needs about 16 r + 6 vector operations per output
+// pixel for the six outputs (262 at radius 16), which makes it VALU-bound at
+// a quarter of the HBM roofline.  Here both passes run on the matrix cores as
+// banded matrix products, and the vector units only split, scale and store.
+//
+// Strip: one wave64 filters 16 centre rows x 64 centre columns (4 blocks of
+// 16 columns); a workgroup puts 4 waves side by side (16 x 256), so that the
+// rows it writes are long runs (the write stream dominates the traffic).  With RC the radius
+// class (8 or 16; lobes zero-padded to it) and K = 16 + 2 RC:
+//   vertical pass, per field block b (16 input columns starting at
+//   x0 - RC + 16 b) and lobe l:
+//       F^T (16 cols x 16 rows) = X^T (16 cols x K input rows) * T_l^T (K x 16)
+//   horizontal pass, per output block o (16 columns starting at x0 + 16 o):
+//       O^T (16 cols x 16 rows) = T_h^T (16 x K field cols) * F^T (K x 16 rows)
+//   with T the band matrices of the lobes.  The vertical product's
+//   accumulator layout (lane l: row l % 16, columns 4 (l / 16) .. +3) is
+//   exactly the layout the horizontal product wants for its B operand when
+//   the K index of the band matrix is permuted to match, so fields never
+//   leave registers.
+// Precision: matrix operands are f16.  A float v is scaled by a power of two
+// and split as hi = f16(v), lo = f16(v - hi); the products hi*hi + hi*lo +
+// lo*hi carry about 22 significant bits (the dropped lo*lo term and the
+// split residual are each below 2^-22 relative), summed in FP32.  Scales
+// (all powers of two): taps of lobe l by mp[l] (largest tap in [2^11,
+// 2^12)); the input of each strip by mq times a factor putting its largest
+// magnitude in [2^12, 2^13), mq <= 1 / (L1 norm * mp) of every lobe, so that
+// the vertical fields stay below 2^13 and no operand leaves the f16 range;
+// each output is multiplied back by mu[k] / (the strip's factor).  Every scale is a power of two, so apart from f16 subnormals
+// (values 2^-16 below a strip's largest) the results do not depend on them.
+// Non-finite inputs make every output of their 16 x 16 block window
+// non-finite (the band products multiply them by zero).
+#pragma once
+
+#include "sepfir.h"
+#include "batch.h"
+
+#include <type_traits>
+
+namespace tucu {
+
+typedef _Float16 v4f16 __attribute__((ext_vector_type(4)));
+typedef _Float16 v8f16 __attribute__((ext_vector_type(8)));
+
+constexpr int MSTRIP_W = 64;              // centre columns per strip
+constexpr int MNB      = MSTRIP_W / 16;   // output blocks per strip
+constexpr int MWAVES   = 4;               // strips (waves) per workgroup, side by side
+constexpr int MTILE_H  = 16;              // centre rows per workgroup tile
+constexpr int MTILE_W  = MSTRIP_W * MWAVES;   // centre columns per workgroup tile
+constexpr int MOPITCH  = MSTRIP_W + 4;    // row pitch (floats) of a wave's output staging area
+constexpr int MT4      = 64;              // tap-run table entries per lobe
+constexpr int MINVALID = int(0x80000000u);   // buffer offset past any image: loads return 0
+
+template <int RC> struct MGeo {
+    static constexpr bool K48 = RC == 16;                 // K = 48 (two 32-entry slices), else 32
+    static constexpr int  KV  = 16 + 2 * RC;              // input rows per strip
+    static constexpr int  NVB = MNB + (K48 ? 2 : 1);      // field blocks per strip
+    static constexpr int  NLD = K48 ? 12 : 8;             // input values per lane per block
+};
+
+// Per-lane operand runs, in LDS: run[(part * 3 + l) * MT4 + t] holds taps
+// t - 32 .. t - 29 of lobe l, times mp[l], as four f16 (part 0 the high
+// half, part 1 the low half).  Tap o is the weight of the input at offset o
+// (row or column) from the centre, zero beyond the lobe: symmetric lobes (0,
+// 2) weigh both sides c[|o|]; the antisymmetric lobe 1 weighs the upper /
+// left side +c[k] and the lower / right side -c[k] (FIRFilter.h:66-176 pairs
+// them the same way).  A lane's operand is one or two runs, so it loads
+// with 8-byte LDS reads and needs no conversion.
+struct MfmaTabs {
+    v4f16 run[2 * 3 * MT4];
+};
+
+template <class A>
+__device__ __forceinline__ float mfma_tap(const A& a, int l, int o)
+{
+    const int k = o < 0 ? -o : o;
+    if (k > 16) return 0.f;
+    const float c = a.cv[l][k];
+    return l != 1 ? c : o == 0 ? 0.f : o < 0 ? c : -c;
+}
+
+template <class A>
+__device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
+{
+    for (int e = threadIdx.x; e < 3 * MT4; e += blockDim.x) {
+        const int l = e / MT4, t = e - l * MT4;
+        v4f16 hi, lo;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float c = mfma_tap(a, l, t + i - 32) * a.mp[l];
+            const _Float16 h = _Float16(c);
+            hi[i] = h;
+            lo[i] = _Float16(c - float(h));
+        }
+        T.run[(0 * 3 + l) * MT4 + t] = hi;
+        T.run[(1 * 3 + l) * MT4 + t] = lo;
+    }
+}
+
+__device__ __forceinline__ v8f16 cat8(v4f16 a, v4f16 b)
+{
+    return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Input values of field block b for lane (g, m): column x0 - RC + 16 b + m,
+// rows y0 - RC + 8 g + j (j < 8) and, for K = 48, y0 - RC + 32 + 4 g + j
+// (j < 4).  Pixels outside the image read as 0 (buffer range check).
+template <int RC, class IN, class A>
+__device__ __forceinline__ void
+mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, int g, int m,
+                bool rows_ok, float* v)
+{
+    constexpr int NLD = MGeo<RC>::NLD;
+    const int es = int(sizeof(IN));
+    const int xb = x0 - RC + 16 * b;
+    const int col = xb + m;
+    const int sb = a.in_stride_b;
+    const int r0 = y0 - RC + 8 * g;            // first row of entries 0 .. 7
+    const int r1 = y0 - RC + 32 + 4 * g;       // first row of entries 8 .. 11
+    if (rows_ok && xb >= 0 && xb + 16 <= a.ncol) {          // uniform: every pixel inside
+        const int v0 = r0 * sb + col * es, v1 = r1 * sb + col * es;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = Px<IN>::ld(rsrc, v0, j * sb);
+        if constexpr (MGeo<RC>::K48) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[8 + j] = Px<IN>::ld(rsrc, v1, j * sb);
+        }
+    } else {
+        const bool colok = unsigned(col) < unsigned(a.ncol);
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int row = j < 8 ? r0 + j : r1 + (j - 8);
+            const bool ok = colok && unsigned(row) < unsigned(a.nrow);
+            v[j] = Px<IN>::ld(rsrc, ok ? row * sb + col * es : MINVALID, 0);
+        }
+    }
+}
+
+// f16 hi / lo split of 4 floats times s.
+__device__ __forceinline__ void split4(float v0, float v1, float v2, float v3, float s, v4f16& hi, v4f16& lo)
+{
+    const v4f32 x = {v0 * s, v1 * s, v2 * s, v3 * s};
+    hi = __builtin_convertvector(x, v4f16);
+    lo = __builtin_convertvector(x - __builtin_convertvector(hi, v4f32), v4f16);
+}
+
+// Entries 4 O .. 4 O + 7 of a field vector: field blocks O and O + 1.
+template <int O, class V>
+__device__ __forceinline__ v8f16 pair8(const V& f)
+{
+    return __builtin_shufflevector(f, f, 4 * O, 4 * O + 1, 4 * O + 2, 4 * O + 3,
+                                   4 * O + 4, 4 * O + 5, 4 * O + 6, 4 * O + 7);
+}
+
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled.
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+#define TUCU_MFMA(A_, B_, C_) __builtin_amdgcn_mfma_f32_16x16x32_f16((A_), (B_), (C_), 0, 0, 0)
+
+// One strip: centre rows y0 .. y0+15, centre columns x0 .. x0+63.  T holds
+// the operand runs (mfma_build_tabs).  Every lane of the wave calls it.
+//
+// Operand layouts (lane = 16 g + m; m is also the row / output column n):
+//   vertical, block b, K32 slice: A = input rows y0 - RC + 8 g + j (j < 8)
+//     of column x0 - RC + 16 b + m; B = taps of offsets 8 g + j - RC - n.
+//   vertical, K48 second slice: A = entries 4 .. 11 of the lane's input,
+//     i.e. rows 8 g + 4 .. 8 g + 7 (weight 0) and 32 + 4 g .. +3; B = zeros
+//     and taps of offsets 32 + 4 g + j - RC - n.
+//   horizontal, output block o, K32 slice: B = field blocks o, o + 1 (lane
+//     holds columns 4 g .. 4 g + 3 of each, window columns 4 g + j and
+//     16 + 4 g + j); A = taps of offsets (window column) - RC - m.
+//   horizontal, K48 second slice: B = field blocks o + 1 (weight 0), o + 2.
+// A strip's input, as loaded: field block b, entry j (mfma_load_block's
+// order); radius class 8 uses blocks 0 .. 4 and entries 0 .. 7.
+typedef float MfmaRaw[6][12];
+
+// Issues the loads of the strip at (y0, x0); the values arrive in raw.
+template <int RC, class IN, class A>
+__device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& raw)
+{
+    using G = MGeo<RC>;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
+    const bool rows_ok = y0 - RC >= 0 && y0 - RC + G::KV <= a.nrow;
+#pragma unroll
+    for (int b = 0; b < G::NVB; ++b) {
+        mfma_load_block<RC, IN>(a, rsrc, y0, x0, b, g, m, rows_ok, raw[b]);
+        if constexpr (!G::K48) raw[b][8] = raw[b][9] = raw[b][10] = raw[b][11] = 0.f;
+    }
+}
+
+// One strip: centre rows y0 .. y0+15, centre columns x0 .. x0+63, input in
+// raw (mfma_load).  T holds the operand runs (mfma_build_tabs), stage is the
+// wave's output staging area.  after_vpass() runs once the input has been
+// consumed (it may reuse raw, e.g. to prefetch the next strip).  Every lane
+// of the wave calls it.
+//
+// Operand layouts (lane = 16 g + m; m is also the row / output column n):
+//   vertical, block b, K32 slice: A = input rows y0 - RC + 8 g + j (j < 8)
+//     of column x0 - RC + 16 b + m; B = taps of offsets 8 g + j - RC - n.
+//   vertical, K48 second slice: A = entries 4 .. 11 of the lane's input,
+//     i.e. rows 8 g + 4 .. 8 g + 7 (weight 0) and 32 + 4 g .. +3; B = zeros
+//     and taps of offsets 32 + 4 g + j - RC - n.
+//   horizontal, output block o, K32 slice: B = field blocks o, o + 1 (lane
+//     holds columns 4 g .. 4 g + 3 of each, window columns 4 g + j and
+//     16 + 4 g + j); A = taps of offsets (window column) - RC - m.
+//   horizontal, K48 second slice: B = field blocks o + 1 (weight 0), o + 2.
+template <int RC, class IN, class OUT, class A, class F>
+__device__ __forceinline__ void
+mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
+{
+    using G = MGeo<RC>;
+    constexpr int NVB = G::NVB, NLD = G::NLD;
+    constexpr bool XLO = sizeof(IN) == 4;       // f16 / bf16 / u8 inputs are exact in one f16
+    constexpr int NF = 4 * NVB;                 // field entries per lane
+    typedef _Float16 vfield __attribute__((ext_vector_type(NF)));
+    typedef _Float16 vin __attribute__((ext_vector_type(12)));
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, m = lane & 15;
+    const unsigned mask = a.mask;
+    const v4f16 z4 = {0, 0, 0, 0};
+
+    float mx = 0.f;
+#pragma unroll
+    for (int b = 0; b < NVB; ++b)
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) mx = fmaxf(mx, fabsf(raw[b][j]));
+    mx = wave_max(mx);
+    int ex;
+    (void)frexpf(mx, &ex);                      // mx < 2^ex
+    ex = 13 - ex;                               // largest scaled input in [2^12, 2^13)
+    ex = ex < -100 ? -100 : ex > 100 ? 100 : ex;
+    const float s = ldexpf(a.mq, ex), inv_s = ldexpf(1.f, -ex);
+
+    vin xh[NVB], xl[NVB];
+#pragma unroll
+    for (int b = 0; b < NVB; ++b) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v4f16 h, l;
+            split4(raw[b][4 * q], raw[b][4 * q + 1], raw[b][4 * q + 2], raw[b][4 * q + 3], s, h, l);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { xh[b][4 * q + i] = h[i]; xl[b][4 * q + i] = l[i]; }
+        }
+    }
+
+    // --------------------------------------------------- vertical pass
+    // fh[l] / fl[l]: entries 4 b .. 4 b + 3 = field l at row y0 + n, columns
+    // x0 - RC + 16 b + 4 g .. +3 (the accumulator layout), split in f16.
+    // One lobe at a time, the chains of the field blocks interleaved.
+    vfield fh[3], fl[3];
+    const unsigned fneed[3] = {mask & 0x0bu, mask & 0x14u, mask & 0x20u};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        if (!fneed[l]) continue;
+        const int t1 = 32 + 8 * g - RC - m;
+        const v8f16 ch1 = cat8(T.run[(0 * 3 + l) * MT4 + t1], T.run[(0 * 3 + l) * MT4 + t1 + 4]);
+        const v8f16 cl1 = cat8(T.run[(1 * 3 + l) * MT4 + t1], T.run[(1 * 3 + l) * MT4 + t1 + 4]);
+        v8f16 ch2, cl2;
+        if constexpr (G::K48) {
+            const int t2 = 64 + 4 * g - RC - m;
+            ch2 = cat8(z4, T.run[(0 * 3 + l) * MT4 + t2]);
+            cl2 = cat8(z4, T.run[(1 * 3 + l) * MT4 + t2]);
+        }
+        v4f32 acc[NVB];
+#pragma unroll
+        for (int b = 0; b < NVB; ++b) acc[b] = v4f32{0.f, 0.f, 0.f, 0.f};
+        // smaller products first
+        if constexpr (XLO) {
+#pragma unroll
+            for (int b = 0; b < NVB; ++b)
+                acc[b] = TUCU_MFMA(__builtin_shufflevector(xl[b], xl[b], 0, 1, 2, 3, 4, 5, 6, 7), ch1, acc[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < NVB; ++b)
+            acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 0, 1, 2, 3, 4, 5, 6, 7), cl1, acc[b]);
+        if constexpr (G::K48) {
+            if constexpr (XLO) {
+#pragma unroll
+                for (int b = 0; b < NVB; ++b)
+                    acc[b] = TUCU_MFMA(__builtin_shufflevector(xl[b], xl[b], 4, 5, 6, 7, 8, 9, 10, 11), ch2, acc[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < NVB; ++b)
+                acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 4, 5, 6, 7, 8, 9, 10, 11), cl2, acc[b]);
+#pragma unroll
+            for (int b = 0; b < NVB; ++b)
+                acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 4, 5, 6, 7, 8, 9, 10, 11), ch2, acc[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < NVB; ++b)
+            acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 0, 1, 2, 3, 4, 5, 6, 7), ch1, acc[b]);
+#pragma unroll
+        for (int b = 0; b < NVB; ++b) {
+            v4f16 h, lo;
+            split4(acc[b][0], acc[b][1], acc[b][2], acc[b][3], 1.f, h, lo);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { fh[l][4 * b + i] = h[i]; fl[l][4 * b + i] = lo[i]; }
+        }
+    }
+
+    after_vpass();
+
+    // ------------------------------------------------- horizontal pass
+    const bool fast = y0 >= a.fy0 && y0 + 16 <= a.fy1 && x0 >= a.fx0 && x0 + MSTRIP_W <= a.fx1;
+    const int y = y0 + m;
+    constexpr int HL[6] = {0, 1, 0, 2, 1, 0};   // horizontal lobe of output k
+    constexpr int VL[6] = {0, 0, 1, 0, 1, 2};   // vertical field of output k
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const int h = HL[k], v = VL[k];
+        const int t1 = 32 + 4 * g - RC - m;
+        const v8f16 ch1 = cat8(T.run[(0 * 3 + h) * MT4 + t1], T.run[(0 * 3 + h) * MT4 + t1 + 16]);
+        const v8f16 cl1 = cat8(T.run[(1 * 3 + h) * MT4 + t1], T.run[(1 * 3 + h) * MT4 + t1 + 16]);
+        v8f16 ch2, cl2;
+        if constexpr (G::K48) {
+            const int t2 = 64 + 4 * g - RC - m;
+            ch2 = cat8(z4, T.run[(0 * 3 + h) * MT4 + t2]);
+            cl2 = cat8(z4, T.run[(1 * 3 + h) * MT4 + t2]);
+        }
+        v4f32 acc[MNB];
+#pragma unroll
+        for (int o = 0; o < MNB; ++o) acc[o] = v4f32{0.f, 0.f, 0.f, 0.f};
+        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(cl1, pair8<o>(fh[v]), acc[o]); });
+        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fl[v]), acc[o]); });
+        if constexpr (G::K48) {
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(cl2, pair8<o + 1>(fh[v]), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch2, pair8<o + 1>(fl[v]), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch2, pair8<o + 1>(fh[v]), acc[o]); });
+        }
+        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fh[v]), acc[o]); });
+        const float u = a.mu[k] * inv_s;
+        if (fast) {
+            // Through the wave's LDS staging area, so that each store writes
+            // 4 rows x 64 contiguous pixels instead of 16 rows x 16.
+#pragma unroll
+            for (int o = 0; o < MNB; ++o)
+                *reinterpret_cast<v4f32*>(stage + m * MOPITCH + 16 * o + 4 * g) = acc[o] * u;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int o = 0; o < MNB; ++o) {
+                const int row = 4 * o + g;
+                const v4f32 q = *reinterpret_cast<const v4f32*>(stage + row * MOPITCH + 4 * m);
+                const gptr<OUT> p = reinterpret_cast<gptr<OUT>>(
+                    a.obase[k] + uintptr_t(int64_t(y0 + row) * a.ostride[k] + x0 + 4 * m) * sizeof(OUT));
+                Px<OUT>::st4(p, q[0], q[1], q[2], q[3]);
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            const int rv = a.r[v], rh = a.r[h];
+#pragma unroll
+            for (int o = 0; o < MNB; ++o)
+                store4<OUT>(a, k, false, rv, a.nrow - rv, rh, a.ncol - rh, y, x0 + 16 * o + 4 * g,
+                            acc[o][0] * u, acc[o][1] * u, acc[o][2] * u, acc[o][3] * u);
+        }
+    }
+}
+#undef TUCU_MFMA
+
+// Workgroup tiles (strip-major: tile = tx * tiles_y + ty) hold WY strips
+// down by 4 / WY across, i.e. 16 WY rows x 256 / WY columns.  Every shape
+// filters the same 16 x 64 strips (at rows cy0 + 16 i, columns cx0 + 64 j),
+// so the shape changes no result.  Position of this wave's strip in `tile`;
+// false if the strip lies outside the region.
+template <int WY, class A>
+__device__ __forceinline__ bool mfma_strip_pos(const A& a, int tile, int& y0, int& x0)
+{
+    constexpr int WX = MWAVES / WY;
+    const int tx = tile / a.tiles_y;
+    const int ty = tile - tx * a.tiles_y;
+    const int w  = int(threadIdx.x) >> 6;
+    y0 = a.cy0 + ty * (16 * WY) + 16 * (w / WX);
+    x0 = a.cx0 + tx * (MSTRIP_W * WX) + MSTRIP_W * (w % WX);
+    return y0 < a.cy1 && x0 < a.cx1;
+}
+
+// One image, one workgroup per tile.
+template <int RC, int WY, class IN, class OUT>
+__global__ void __launch_bounds__(64 * MWAVES)
+sepfir_mfma_kernel(const SepArgs a)
+{
+    __shared__ MfmaTabs T;
+    __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    mfma_build_tabs(a, T);
+    __syncthreads();
+    int y0, x0;
+    if (!mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0)) return;
+    MfmaRaw raw;
+    mfma_load<RC, IN>(a, y0, x0, raw);
+    mfma_compute<RC, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH,
+                              [] {});
+}
+
+template <int RC, int WY, class IN, class OUT>
+hipError_t launch_sepfir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((sepfir_mfma_kernel<RC, WY, IN, OUT>), dim3(nblocks), dim3(64 * MWAVES), 0, s, a);
+    return hipGetLastError();
+}
+
+// Matrix-core variant of batch_kernel: the same tile scheduling; the tap
+// tables in LDS are rebuilt when the job changes (tiles of a job are
+// contiguous in the list).
+template <int WY, class IN, class OUT>
+__global__ void __launch_bounds__(64 * MWAVES)
+batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict__ tiles, int ntiles,
+                  int* __restrict__ counter)
+{
+    __shared__ MfmaTabs T;
+    __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    __shared__ int next;
+    ConstArgs* const jobs = (ConstArgs*)(uintptr_t)jobs_;
+    float* const stage = reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH;
+    int cur = -1;
+    int ti = int(blockIdx.x);
+    while (ti < ntiles) {
+        if (threadIdx.x == 0)          // the answer is read after this tile
+            next = int(gridDim.x) + __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const BatchTile e = tiles[ti];
+        ConstArgs& a = jobs[e.job];
+        if (int(e.job) != cur) {       // uniform over the workgroup
+            mfma_build_tabs(a, T);
+            cur = int(e.job);
+            __syncthreads();
+        }
+        int y0, x0;
+        if (mfma_strip_pos<WY>(a, int(e.tile), y0, x0)) {
+            MfmaRaw raw;
+            if (a.rt <= 8) {
+                mfma_load<8, IN>(a, y0, x0, raw);
+                mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
+            } else {
+                mfma_load<16, IN>(a, y0, x0, raw);
+                mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
+            }
+        }
+        __syncthreads();               // T reuse; `next` was written before this barrier
+        ti = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();               // all have read `next` before thread 0 writes it again
+    }
+}
+
+template <int WY, class IN, class OUT>
+hipError_t launch_batch_mfma(const SepArgs* jobs, const BatchTile* tiles, int ntiles, int* counter,
+                             int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_counter<1>, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL((batch_mfma_kernel<WY, IN, OUT>), dim3(grid), dim3(64 * MWAVES), 0, s, jobs, tiles, ntiles, counter);
+    return hipGetLastError();
+}
+
+// Workgroup shapes: WY = 1, 2, 4 strips down (tile rows 16, 32, 64).
+constexpr int MFMA_RADII[2] = {8, 16};
+constexpr int NMWY = 3;
+constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
+constexpr int MFMA_WY_DEFAULT = 1;          // index: 2 x 2 strips, 32 x 128 pixels
+extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
+
+}   // namespace tucu

@@ -78,6 +78,8 @@ struct SepArgs {
     float    ch[KMAX];             // FIR: horizontal lobe
     float    tv[TMAX], th[TMAX];   // TAPS: full vertical / horizontal taps, tap k at offset k - R
     int      rt;                   // template radius class the tile grid was laid out for (batches)
+    float    mp[3], mq;            // matrix-core kernels: power-of-two scales of each lobe's taps
+    float    mu[6];                // and of the input, and the factor that undoes them per output
     uintptr_t obase[6];            // output k: address of the element for centre (0, 0), i.e.
                                    // data + (dy_k * stride + dx_k) * elem (may lie before data)
     int32_t  ostride[6];           // output row stride, elements
@@ -450,10 +452,14 @@ sepfir_kernel(const SepArgs a)
 // Host-side launch entry of one instantiation.
 using LaunchFn = hipError_t (*)(const SepArgs&, int nblocks, hipStream_t);
 
+// Every member has an initializer, so the tables below are constant-
+// initialized before any registrar (a static constructor in another unit)
+// fills them.
 struct KernelEntry {
-    LaunchFn fn;
-    int      th;     // tile rows
-    int      rv, rh; // template radii
+    LaunchFn fn = nullptr;
+    int      th = 0;          // tile rows
+    int      rv = 0, rh = 0;  // template radii
+    int      tw = 0;          // tile columns (0: NT - 2 rh)
 };
 
 template <int RV, int RH, int TH, int MODE, class IN, class OUT>

@@ -237,14 +237,15 @@ bool reap(tucu_server_s& s)
 // true if a batch was launched.
 bool admit_and_launch(tucu_server_s& s, Slot& sl)
 {
-    const BatchEntry& e = g_batch_table[s.thi][s.cfg.in_dtype][s.cfg.out_dtype];
+    const BatchEntry& e = batch_entry(s.thi, s.cfg.in_dtype, s.cfg.out_dtype);
+    const int tile_w = e.tw ? e.tw : NT - 32;
     std::vector<SepArgs> jobs;
     std::vector<Request*> reqs;
     size_t tiles_est = 0;
     while (!s.waiting.empty() && int(jobs.size()) < s.cfg.max_batch_jobs) {
         Request* r = s.waiting.front();
         // tiles at the largest radius (16) and one extra row and column, an upper bound
-        const size_t est = size_t((r->req.in.nrow + e.th - 1) / e.th + 1) * size_t(r->req.in.ncol / (NT - 32) + 2);
+        const size_t est = size_t((r->req.in.nrow + e.th - 1) / e.th + 1) * size_t(r->req.in.ncol / tile_w + 2);
         if (!jobs.empty() && tiles_est + est > size_t(s.cfg.max_tiles)) break;
         if (!allocate_outputs(s, r)) break;                   // admission by free HBM: FIFO, head blocks
         const Lobes& l = lobes_for(s, r->req.sigma);
@@ -266,7 +267,7 @@ bool admit_and_launch(tucu_server_s& s, Slot& sl)
     }
     if (jobs.empty()) return false;
     std::vector<BatchTile> tiles;
-    int rc = make_tiles(jobs, e.th, tiles);
+    int rc = make_tiles(jobs, e, tiles);
     if (rc == TUCU_OK && tiles.size() > size_t(s.cfg.max_tiles)) rc = TUCU_ERR_INVALID;
     hipError_t err = hipSuccess;
     if (rc == TUCU_OK) {
@@ -353,9 +354,10 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
     if (dtype_size(cfg->in_dtype) == 0 || dtype_size(cfg->out_dtype) == 0)
         return fail(TUCU_ERR_INVALID, "unknown dtype");
     int thi = -1;
-    const int rows = cfg->tile_rows > 0 ? cfg->tile_rows : BATCH_THS[BATCH_TH_DEFAULT];
+    const int rows = cfg->tile_rows > 0 ? cfg->tile_rows : batch_entry(BATCH_TH_DEFAULT, cfg->in_dtype, cfg->out_dtype).th;
     for (int i = 0; i < NBTH; ++i)
-        if (BATCH_THS[i] == rows && g_batch_table[i][cfg->in_dtype][cfg->out_dtype].fn) thi = i;
+        if (batch_entry(i, cfg->in_dtype, cfg->out_dtype).th == rows && batch_entry(i, cfg->in_dtype, cfg->out_dtype).fn)
+            thi = i;
     if (thi < 0) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
     if (cfg->pool_bytes == 0) return fail(TUCU_ERR_INVALID, "empty output pool");
     auto s = new tucu_server_s();

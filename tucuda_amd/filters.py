@@ -233,6 +233,15 @@ class FIRGaussianConvolver2(FIRFilter2):
             ctypes.byref(src), descs, mask, int(bool(shift)), _stream_handle(inp)))
 
 
+def batch_tile_rows():
+    """Tile heights the batched kernels offer: the matrix-core kernels
+    (default) take 16, 32 or 64 rows (workgroups of 4 x 1, 2 x 2 or 1 x 4
+    strips of 16 x 64 pixels), the direct FP32 kernels (TUCU_GAUSS_PATH=valu)
+    4, 8 or 16."""
+    import os
+    return (4, 8, 16) if os.environ.get("TUCU_GAUSS_PATH") == "valu" else (16, 32, 64)
+
+
 class GaussBatch:
     """Batched mode (extension): many Gaussian jobs of uneven sizes and
     sigmas in one persistent launch.  ``jobs`` is a sequence of

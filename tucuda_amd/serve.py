@@ -23,7 +23,7 @@ from .filters import _DTYPES, image_desc
 
 class Server:
     def __init__(self, pool_bytes: int, in_dtype=torch.bfloat16, out_dtype=torch.bfloat16,
-                 rule: int = _capi.TUCU_RULE_CEIL4SIGMA, tile_rows: int = 8, max_batch_jobs: int = 1024,
+                 rule: int = _capi.TUCU_RULE_CEIL4SIGMA, tile_rows: int = 0, max_batch_jobs: int = 1024,
                  max_inflight: int = 2, max_tiles: int = 0):
         cfg = TucuServerConfig(int(pool_bytes), _DTYPES[in_dtype], _DTYPES[out_dtype], int(rule),
                                int(tile_rows), int(max_batch_jobs), int(max_inflight), int(max_tiles), 0)

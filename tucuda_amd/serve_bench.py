@@ -277,7 +277,7 @@ def main(argv=None):
     ap.add_argument("--images", type=int, default=256, help="resident input images")
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--pool-gib", type=float, default=16.0)
-    ap.add_argument("--tile-rows", type=int, default=8)
+    ap.add_argument("--tile-rows", type=int, default=0, help="batched tile rows (0: the library default)")
     ap.add_argument("--max-inflight", type=int, default=2)
     ap.add_argument("--capacity-requests", type=int, default=2000)
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
