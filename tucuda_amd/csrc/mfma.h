@@ -361,7 +361,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
                 const v4f32 q = *reinterpret_cast<const v4f32*>(stage + row * MOPITCH + 4 * m);
                 const gptr<OUT> p = reinterpret_cast<gptr<OUT>>(
                     a.obase[k] + uintptr_t(int64_t(y0 + row) * a.ostride[k] + x0 + 4 * m) * sizeof(OUT));
-                Px<OUT>::st4(p, q[0], q[1], q[2], q[3]);
+                Px<OUT>::st4nt(p, q[0], q[1], q[2], q[3]);    // written once: streaming stores
             }
             __builtin_amdgcn_wave_barrier();
         } else {

@@ -104,6 +104,10 @@ template <> struct Px<float> {
     {
         *reinterpret_cast<gptr<v4f32>>(p) = v4f32{a, b, c, d};
     }
+    static __device__ __forceinline__ void st4nt(gptr<float> p, float a, float b, float c, float d)
+    {
+        __builtin_nontemporal_store(v4f32{a, b, c, d}, reinterpret_cast<gptr<v4f32>>(p));
+    }
 };
 
 template <> struct Px<uint8_t> {
@@ -121,6 +125,10 @@ template <> struct Px<uint8_t> {
     {
         *reinterpret_cast<gptr<uint32_t>>(p) = cvt(a) | cvt(b) << 8 | cvt(c) << 16 | cvt(d) << 24;
     }
+    static __device__ __forceinline__ void st4nt(gptr<uint8_t> p, float a, float b, float c, float d)
+    {
+        __builtin_nontemporal_store(cvt(a) | cvt(b) << 8 | cvt(c) << 16 | cvt(d) << 24, reinterpret_cast<gptr<uint32_t>>(p));
+    }
 };
 
 template <> struct Px<f16_t> {
@@ -132,6 +140,10 @@ template <> struct Px<f16_t> {
     static __device__ __forceinline__ void st4(gptr<f16_t> p, float a, float b, float c, float d)
     {
         *reinterpret_cast<gptr<v2u32>>(p) = v2u32{pack_f16(a, b), pack_f16(c, d)};
+    }
+    static __device__ __forceinline__ void st4nt(gptr<f16_t> p, float a, float b, float c, float d)
+    {
+        __builtin_nontemporal_store(v2u32{pack_f16(a, b), pack_f16(c, d)}, reinterpret_cast<gptr<v2u32>>(p));
     }
 };
 
@@ -148,6 +160,10 @@ template <> struct Px<bf16_t> {
     {
         *reinterpret_cast<gptr<v2u32>>(p) = v2u32{pack_bf16(a, b), pack_bf16(c, d)};
     }
+    static __device__ __forceinline__ void st4nt(gptr<bf16_t> p, float a, float b, float c, float d)
+    {
+        __builtin_nontemporal_store(v2u32{pack_bf16(a, b), pack_bf16(c, d)}, reinterpret_cast<gptr<v2u32>>(p));
+    }
 };
 
 // Results of centre columns xc .. xc+3 of centre row y for output k, whose
@@ -158,8 +174,8 @@ store4(const A& a, int k, bool fast, int vy0, int vy1, int vx0, int vx1,
        int y, int xc, float v0, float v1, float v2, float v3)
 {
     const gptr<OUT> q = reinterpret_cast<gptr<OUT>>(a.obase[k] + uintptr_t(int64_t(y) * a.ostride[k] + xc) * sizeof(OUT));
-    if (fast) {
-        Px<OUT>::st4(q, v0, v1, v2, v3);
+    if (fast) {                                 // outputs are written once: streaming stores
+        Px<OUT>::st4nt(q, v0, v1, v2, v3);
         return;
     }
     if (y < vy0 || y >= vy1) return;
