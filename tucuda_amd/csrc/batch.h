@@ -72,6 +72,7 @@ struct BatchEntry {
     const void*   kernel = nullptr;   // for occupancy queries
     int           th = 0;             // tile rows
     int           tw = 0;             // tile columns (0: NT - 2 x radius class)
+    int           nthreads = NT;      // workgroup size
 };
 
 template <int TH, class IN, class OUT>

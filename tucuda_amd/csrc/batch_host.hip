@@ -56,7 +56,7 @@ int batch_grid(const BatchEntry& e, int ntiles)
     int dev = 0, ncu = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, NT, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, e.nthreads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     return std::max(1, std::min(ntiles, std::max(1, ncu) * per_cu));
 }

@@ -186,6 +186,17 @@ int finish_args(SepArgs& a, const tucu_image& in, unsigned live, const tucu_imag
     a.in_stride_b = int32_t(stride_b);
     a.in_bytes = uint32_t(int64_t(in.nrow - 1) * stride_b + in.ncol * es);
     a.nrow = in.nrow; a.ncol = in.ncol;
+    // Start the column grid on a 128-byte boundary of the outputs at or before
+    // the written region (possibly left of the image; tiles mask the columns
+    // outside, and their loads read zeros there): tiles then write whole
+    // cache lines and load aligned segments, whatever the radius.
+    for (int k = 0; k < 6; ++k) {
+        if (!(live & (1u << k))) continue;
+        const int oes = int(dtype_size(outs[k].dtype));
+        for (int d = 0; d < 128 / oes; ++d)
+            if ((a.obase[k] + uintptr_t(intptr_t(a.cx0 - d) * oes)) % 128 == 0) { a.cx0 -= d; break; }
+        break;                                  // the first live output sets the phase
+    }
     a.fy0 = a.fx0 = 0; a.fy1 = in.nrow; a.fx1 = in.ncol;
     bool aligned = true;
     for (int k = 0; k < 6; ++k) {

@@ -458,6 +458,58 @@ batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict
     }
 }
 
+// Batched, one wave per workgroup: a tile is one strip (16 x 64; tile =
+// tx * tiles_y + ty), every wave draws its own strips from the counter, so
+// the waves of a SIMD drift apart and their loads, matrix work and stores
+// overlap.  Each wave keeps its own tap tables.
+template <class IN, class OUT>
+__global__ void __launch_bounds__(64)
+batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict__ tiles, int ntiles,
+                       int* __restrict__ counter)
+{
+    __shared__ MfmaTabs T;
+    __shared__ v4f32 stage4[16 * MOPITCH / 4];
+    ConstArgs* const jobs = (ConstArgs*)(uintptr_t)jobs_;
+    int cur = -1;
+    int ti = int(blockIdx.x);
+    while (ti < ntiles) {
+        int nxt = 0;
+        if (threadIdx.x == 0)          // the answer is read after this strip
+            nxt = int(gridDim.x) + __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const BatchTile e = tiles[ti];
+        ConstArgs& a = jobs[e.job];
+        if (int(e.job) != cur) {       // LDS accesses of one wave run in order
+            mfma_build_tabs(a, T);
+            cur = int(e.job);
+            __builtin_amdgcn_wave_barrier();
+        }
+        const int tx = int(e.tile) / a.tiles_y;
+        const int y0 = a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y);
+        const int x0 = a.cx0 + MSTRIP_W * tx;
+        if (y0 < a.cy1 && x0 < a.cx1) {
+            MfmaRaw raw;
+            if (a.rt <= 8) {
+                mfma_load<8, IN>(a, y0, x0, raw);
+                mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+            } else {
+                mfma_load<16, IN>(a, y0, x0, raw);
+                mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        ti = __builtin_amdgcn_readfirstlane(nxt);
+    }
+}
+
+template <class IN, class OUT>
+hipError_t launch_batch_mfma_wave(const SepArgs* jobs, const BatchTile* tiles, int ntiles, int* counter,
+                                  int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(reset_counter<1>, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL((batch_mfma_wave_kernel<IN, OUT>), dim3(grid), dim3(64), 0, s, jobs, tiles, ntiles, counter);
+    return hipGetLastError();
+}
+
 template <int WY, class IN, class OUT>
 hipError_t launch_batch_mfma(const SepArgs* jobs, const BatchTile* tiles, int ntiles, int* counter,
                              int grid, hipStream_t s)
