@@ -67,6 +67,39 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0):
             "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)"}
 
 
+def write_fill_gbs(dev, nbytes: int, nbuf: int = 10, reps: int = 5) -> float:
+    """HBM write rate of a plain fill of nbytes (one kernel per buffer,
+    nbuf rotating buffers so the Infinity Cache cannot hold them), replayed
+    from a HIP graph: the rate a write stream of the outputs' size reaches
+    with no arithmetic."""
+    import torch
+    bufs = [torch.empty(nbytes // 4, dtype=torch.float32, device=dev) for _ in range(nbuf)]
+    for b in bufs:
+        b.fill_(0.0)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    st = torch.cuda.Stream(dev)
+    st.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(st):
+        with torch.cuda.graph(graph, stream=st):
+            for b in bufs:
+                b.fill_(1.0)
+    torch.cuda.current_stream(dev).wait_stream(st)
+    graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rs = torch.cuda.current_stream(dev)
+    e0.record(rs)
+    for _ in range(reps):
+        graph.replay()
+    e1.record(rs)
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / (reps * nbuf)
+    del graph, bufs
+    torch.cuda.empty_cache()
+    return nbytes / sec / 1e9
+
+
 def served_leg(dev, rates: str, seconds: float) -> dict:
     """p50/p99 latency of the continuous-batching server under Poisson
     arrivals: 256 resident bfloat16 images (20% 3840x2160, the rest
@@ -275,6 +308,11 @@ def main():
             "algorithmic_bytes": alg_bytes,
         },
     }
+    # Context for `frac`: the rate a plain fill of the six outputs' bytes
+    # reaches (the outputs are 85% of the kernel's bytes).
+    fill = write_fill_gbs(dev, 6 * nrow * ncol * 4)
+    line["roofline"]["write_fill_gbs"] = round(fill, 1)
+    log(f"plain fill of {6 * nrow * ncol * 4 / 1e6:.1f} MB: {fill:.0f} GB/s")
     if args.serve_seconds > 0:
         line["served"] = served_leg(dev, args.serve_rates, args.serve_seconds)
     if not args.no_cpu:

@@ -5,7 +5,7 @@ Workloads: the 4096x4096 u_char image (seed 2) filtered by FIRFilter2 with
 N(0,1) lobes normalised to unit L1 norm of the full kernel -- symmetric
 9 / 5 (17 x 9 taps), antisymmetric 8 / 4, and non-symmetric 17 x 9 general
 taps -- into float32 and float16; and the six Gaussian outputs with BF16
-storage.  Errors are normwise: max|got - ref| / max|ref| and
+and with FP32 storage.  Errors are normwise: max|got - ref| / max|ref| and
 rms(got - ref) / rms(ref), with ref the FP64 oracle on the exact input."""
 import json
 import os
@@ -130,6 +130,28 @@ def test_bf16_storage_gauss6(dev, sigma):
                 assert errs["max_rel"] <= 1e-2, (m, name, errs)
             else:
                 assert errs["rms_rel"] <= 1e-2, (m, name, errs)
+
+
+@pytest.mark.parametrize("sigma", [0.8, 2.0, 3.0, 4.0])
+def test_f32_gauss6_against_fp64(dev, sigma):
+    """FP32 storage through the default six-output kernel (matrix cores,
+    f16 hi / lo operand splits, FP32 accumulation) against the FP64 oracle:
+    normwise max error <= 1e-5, the FP32 parity bar, and recorded."""
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(4, 1080, 1920)
+    lobes64 = oracle.gauss_lobes(sigma, "f64")
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    outs = {m: torch.zeros(img.shape, dtype=torch.float32, device=dev) for m in oracle.METHOD_ORDER}
+    g.convolve6(to_dev(img, dev), outs)
+    for m in oracle.METHOD_ORDER:
+        h, v = oracle.METHODS[m]
+        ref = np.full(img.shape, np.nan, np.float32)
+        oracle.fir2(lobes64[h], lobes64[v], img, ref, True, "f64", nthreads=16)
+        valid = ~np.isnan(ref)
+        errs = _errors(to_host(outs[m])[valid], ref[valid].astype(np.float64))
+        _record(f"gauss6_1080p_f32_sigma{sigma}_{m}", errs)
+        assert errs["max_rel"] <= 1e-5, (m, errs)
 
 
 def test_fp16_storage_matches_oracle_storage_rounding(dev):

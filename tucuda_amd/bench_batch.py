@@ -11,6 +11,8 @@ bfloat16 storage.  Compared forms, all on the same resident buffers:
 * ``per_method``: six FIRGaussianConvolver2 method calls per image, the way
   the reference's API is used (FIRGaussianConvolver.h:178-242); reference-
   rule lobes, since FIRFilter2 only takes the reference's lobe lengths;
+  both timed as issued from Python (``*_ms``, launch overhead included) and
+  replayed from one captured HIP graph (``*_graph_ms``, no host cost);
 * ``batched``: one persistent launch over every image, tile height picked by
   the autotuner, replayed from a captured HIP graph.
 
@@ -70,6 +72,21 @@ def main(argv=None):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    def captured(fn):
+        """fn's launches captured in one HIP graph; returns its replay."""
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        st = torch.cuda.Stream(dev)
+        st.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(st):
+            with torch.cuda.graph(g, stream=st):
+                fn()
+        torch.cuda.current_stream(dev).wait_stream(st)
+        keep.append(g)
+        return g.replay
+
+    keep = []
     for dname in args.dtypes.split(","):
         dt = getattr(torch, dname)
         ins, outs = [], []
@@ -84,6 +101,7 @@ def main(argv=None):
                 g.convolve6(x, o)
 
         ms_plain = timed(plain, max(1, args.reps // 4))
+        ms_plain_graph = timed(captured(plain), max(1, args.reps // 4))
 
         t0 = time.perf_counter()
         batch = tucuda_amd.GaussBatch([(x, o, g._lobes()) for g, x, o in zip(gs, ins, outs)])
@@ -102,7 +120,9 @@ def main(argv=None):
             "plain_ms": round(ms_plain, 3),
             "batched_ms": round(ms_batch, 3),
             "batched_eager_ms": round(ms_batch_eager, 3),
+            "plain_graph_ms": round(ms_plain_graph, 3),
             "plain_gpix_s": round(total_px / ms_plain / 1e6, 2),
+            "plain_graph_gpix_s": round(total_px / ms_plain_graph / 1e6, 2),
             "batched_gpix_s": round(total_px / ms_batch / 1e6, 2),
             "speedup_vs_plain": round(ms_plain / ms_batch, 2),
             "tile_rows": rows, "tiles": batch.info()["tiles"], "grid": batch.info()["grid"],
@@ -117,17 +137,23 @@ def main(argv=None):
                     g.diffHH(x, o[3]); g.diffHV(x, o[4]); g.diffVV(x, o[5])
 
             ms_pm = timed(per_method, max(1, args.reps // 4))
+            ms_pm_graph = timed(captured(per_method), max(1, args.reps // 4))
             form["per_method_ms"] = round(ms_pm, 3)
+            form["per_method_graph_ms"] = round(ms_pm_graph, 3)
             form["per_method_gpix_s"] = round(total_px / ms_pm / 1e6, 2)
+            form["per_method_graph_gpix_s"] = round(total_px / ms_pm_graph / 1e6, 2)
         result["forms"][dname] = form
         del batch, graph, ins, outs
+        keep.clear()
         torch.cuda.empty_cache()
     f32 = result["forms"].get("float32", {})
     bf = result["forms"].get("bfloat16", {})
     if f32 and bf:
         result["bf16_batched_vs_f32_plain"] = round(f32["plain_ms"] / bf["batched_ms"], 2)
+        result["bf16_batched_vs_f32_plain_graph"] = round(f32["plain_graph_ms"] / bf["batched_ms"], 2)
         if "per_method_ms" in f32:
             result["bf16_batched_vs_f32_per_method"] = round(f32["per_method_ms"] / bf["batched_ms"], 2)
+            result["bf16_batched_vs_f32_per_method_graph"] = round(f32["per_method_graph_ms"] / bf["batched_ms"], 2)
     print(json.dumps(result), flush=True)
 
 
