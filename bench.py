@@ -100,6 +100,48 @@ def write_fill_gbs(dev, nbytes: int, nbuf: int = 10, reps: int = 5) -> float:
     return nbytes / sec / 1e9
 
 
+def sigma_sweep(dev, nrow: int, ncol: int, sigmas=(0.8, 1.2, 1.6, 2.0, 2.4, 2.8, 3.2, 4.0)) -> dict:
+    """Microseconds per fused six-output launch (radius ceil(4 sigma)) for
+    each sigma, float32 and bfloat16 images, replayed from a HIP graph over
+    6 rotating buffer sets."""
+    import torch
+    import tucuda_amd
+    res = {"workload": f"{ncol}x{nrow}, six outputs, radius ceil(4 sigma), graph replay", "us": {}}
+    for dt in (torch.float32, torch.bfloat16):
+        xs = [torch.rand(nrow, ncol, device=dev).mul(255).to(dt) for _ in range(6)]
+        outs = [[torch.empty(nrow, ncol, device=dev, dtype=dt) for _ in range(6)] for _ in range(6)]
+        for sigma in sigmas:
+            g = tucuda_amd.FIRGaussianConvolver2(sigma, rule=tucuda_amd.TUCU_RULE_CEIL4SIGMA)
+
+            def run():
+                for x, o in zip(xs, outs):
+                    g.convolve6(x, o)
+
+            run()
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            st = torch.cuda.Stream(dev)
+            st.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(st):
+                with torch.cuda.graph(graph, stream=st):
+                    run()
+            torch.cuda.current_stream(dev).wait_stream(st)
+            graph.replay()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            rs = torch.cuda.current_stream(dev)
+            e0.record(rs)
+            for _ in range(30):
+                graph.replay()
+            e1.record(rs)
+            torch.cuda.synchronize()
+            res["us"][f"{str(dt).split('.')[-1]} sigma={sigma}"] = round(e0.elapsed_time(e1) / 30 / 6 * 1000, 2)
+            del graph
+        del xs, outs
+        torch.cuda.empty_cache()
+    return res
+
+
 def served_leg(dev, rates: str, seconds: float) -> dict:
     """p50/p99 latency of the continuous-batching server under Poisson
     arrivals: 256 resident bfloat16 images (20% 3840x2160, the rest
@@ -145,6 +187,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="seconds of Poisson arrivals per served-stream rate (0 skips the served leg)")
     ap.add_argument("--serve-rates", default="200,1000,4000", help="served-stream arrival rates, req/s")
+    ap.add_argument("--sigma-sweep", action="store_true",
+                    help="instead: time one fused six-output 1080p launch for several sigmas "
+                         "(radius ceil(4 sigma)), float32 and bfloat16, and print one JSON line")
     ap.add_argument("--nrow", type=int, default=NROW, help="image rows (the headline workload is 1080)")
     ap.add_argument("--ncol", type=int, default=NCOL, help="image columns (the headline workload is 1920)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -159,6 +204,10 @@ def main():
     import torch
     import tucuda_amd
     from tucuda_amd import synth
+
+    if args.sigma_sweep:
+        print(json.dumps(sigma_sweep(torch.device("cuda:0"), args.nrow, args.ncol)), flush=True)
+        return
 
     dev = torch.device("cuda:0")
     nrow, ncol = args.nrow, args.ncol
