@@ -18,3 +18,17 @@ def dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU visible")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(params=["mfma", "valu"])
+def kernel_family(request, monkeypatch):
+    """Runs a test once on the matrix-core kernels (the default) and once on
+    the direct FP32 vector kernels; the library reads the switches at every
+    call."""
+    if request.param == "valu":
+        monkeypatch.setenv("TUCU_GAUSS_PATH", "valu")
+        monkeypatch.setenv("TUCU_FIR_PATH", "valu")
+    else:
+        monkeypatch.delenv("TUCU_GAUSS_PATH", raising=False)
+        monkeypatch.delenv("TUCU_FIR_PATH", raising=False)
+    return request.param

@@ -43,7 +43,7 @@ def _run(dev, lobeH, lobeV, img, out_np_dtype, shift, out_shape=None):
 
 
 @pytest.mark.parametrize("nH,nV", list(itertools.product(SIZES, SIZES)))
-def test_fir_all_lobe_pairs(dev, nH, nV):
+def test_fir_all_lobe_pairs(dev, kernel_family, nH, nV):
     rng = np.random.default_rng(1000 + 17 * nH + nV)
     lobeH, lobeV = random_lobe(rng, nH), random_lobe(rng, nV)
     # 300 columns = more than one 256-wide strip at every radius; odd rows.
@@ -58,7 +58,7 @@ def test_fir_all_lobe_pairs(dev, nH, nV):
 
 
 @pytest.mark.parametrize("nrow,ncol", [(33, 33), (34, 35), (35, 34), (64, 257), (129, 481), (5, 700)])
-def test_fir_ragged_sizes(dev, nrow, ncol):
+def test_fir_ragged_sizes(dev, kernel_family, nrow, ncol):
     """Sizes just above the no-op threshold, odd sizes, strip boundaries."""
     rng = np.random.default_rng(nrow * 1000 + ncol)
     for nH, nV in [(17, 17), (16, 9), (3, 2), (9, 16), (5, 4)]:
@@ -68,7 +68,7 @@ def test_fir_ragged_sizes(dev, nrow, ncol):
         assert_f32_parity(got, ref, f"{nrow}x{ncol} nH={nH} nV={nV}")
 
 
-def test_fir_too_small_is_noop(dev):
+def test_fir_too_small_is_noop(dev, kernel_family):
     """nrow <= 2 rV or ncol <= 2 rH: nothing is written (FIRFilter.h:304-312)."""
     lobe = random_lobe(np.random.default_rng(0), 17)
     for nrow, ncol in [(32, 100), (100, 32), (10, 10)]:
@@ -98,7 +98,7 @@ def test_fir_errors(dev):
         f.convolve(d_in, small, True)
 
 
-def test_fir_strided_views(dev):
+def test_fir_strided_views(dev, kernel_family):
     """Row ranges with a stride larger than the row (slices of a bigger
     buffer), as cu::Array2 rows padded to 256 bytes are."""
     import torch
@@ -120,7 +120,7 @@ def test_fir_strided_views(dev):
 
 
 @pytest.mark.parametrize("nH,nV", [(1, 1), (3, 5), (17, 9), (9, 17), (33, 3), (5, 33), (11, 25)])
-def test_general_taps(dev, nH, nV):
+def test_general_taps(dev, kernel_family, nH, nV):
     """General-taps extension against the oracle, every radius class and the
     edges of the tap-count range; u8 and float inputs, shift both ways."""
     import tucuda_amd

@@ -25,7 +25,7 @@ def _oracle_all(sigma, img, out_dtype=np.float32, shift=True):
 
 
 @pytest.mark.parametrize("sigma", SIGMAS)
-def test_methods_one_by_one(dev, sigma):
+def test_methods_one_by_one(dev, kernel_family, sigma):
     import tucuda_amd
     img = synth.edge_image(11, 97, 301)
     refs = _oracle_all(sigma, img)
@@ -39,7 +39,7 @@ def test_methods_one_by_one(dev, sigma):
 
 @pytest.mark.parametrize("sigma", SIGMAS)
 @pytest.mark.parametrize("shift", [True, False])
-def test_fused_six(dev, sigma, shift):
+def test_fused_six(dev, kernel_family, sigma, shift):
     import torch
     import tucuda_amd
     img = synth.edge_image(12, 131, 517)
@@ -52,7 +52,7 @@ def test_fused_six(dev, sigma, shift):
 
 
 @pytest.mark.parametrize("mask", [0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x05, 0x3A, 0x24])
-def test_fused_subsets(dev, mask):
+def test_fused_subsets(dev, kernel_family, mask):
     """Any subset of outputs; outputs not asked for are not touched."""
     import torch
     import tucuda_amd
@@ -71,7 +71,7 @@ def test_fused_subsets(dev, mask):
             assert (got == sentinel(np.float32)).all()
 
 
-def test_u8_in_u8_out(dev):
+def test_u8_in_u8_out(dev, kernel_family):
     """gaussTest's instantiation: u_char in, float intermediate, u_char out
     (Testprog/gaussTest/main.cu:62-64)."""
     import tucuda_amd
@@ -97,7 +97,7 @@ def test_workload_640x480_smooth(dev):
     assert_f32_parity(to_host(d_out), ref, "640x480 smooth")
 
 
-def test_workload_1080p_six(dev):
+def test_workload_1080p_six(dev, kernel_family):
     """1920x1080 float32 edge image, seed 1, sigma 3, all six outputs in one
     call against the oracle."""
     import torch

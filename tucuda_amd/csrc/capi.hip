@@ -24,6 +24,7 @@ KernelEntry g_fir_table[4][4][4][4];
 KernelEntry g_taps_table[4][4][4][4];
 KernelEntry g_gauss_table[5][NGTH][4][4];
 KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
+KernelEntry g_fir_mfma_table[2][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
@@ -74,15 +75,47 @@ int gauss_class(int r)
 
 // Tile-height choice of the fused Gaussian kernel: TUCU_GAUSS_TH (one of
 // GAUSS_THS) overrides the default.
-// Six-output Gaussian kernel family: matrix cores (default) or the direct
-// FP32 vector kernel (TUCU_GAUSS_PATH=valu).
+// Kernel families: matrix cores (default) or the direct FP32 vector kernel,
+// TUCU_GAUSS_PATH=valu for the six-output Gaussian (single, batched,
+// served), TUCU_FIR_PATH=valu for FIRFilter2 and general taps.  Read at
+// every call.
 bool gauss_use_mfma()
 {
-    static const bool on = [] {
-        const char* e = std::getenv("TUCU_GAUSS_PATH");
-        return !(e && std::strcmp(e, "valu") == 0);
-    }();
-    return on;
+    const char* e = std::getenv("TUCU_GAUSS_PATH");
+    return !(e && std::strcmp(e, "valu") == 0);
+}
+
+bool fir_use_mfma()
+{
+    const char* e = std::getenv("TUCU_FIR_PATH");
+    return !(e && std::strcmp(e, "valu") == 0);
+}
+
+int launch(const KernelEntry& k, SepArgs& a, hipStream_t s);
+
+// One-output matrix-core launch: full taps vt / ht (tap o at index o + 16)
+// go to a.tv / a.th with their power-of-two scales (mfma.h, MODE_TAPS).
+int launch_fir_mfma(SepArgs& a, const float* vt, const float* ht, int rV, int rH, int in_dt, int out_dt,
+                    hipStream_t s)
+{
+    std::memcpy(a.tv, vt, sizeof(a.tv));
+    std::memcpy(a.th, ht, sizeof(a.th));
+    int e[2] = {0, 0}, eq = 0;
+    for (int l = 0; l < 2; ++l) {
+        const float* t = l == 0 ? vt : ht;
+        float cmax = 0.f, l1 = 0.f;
+        for (int i = 0; i < TMAX; ++i) { cmax = std::max(cmax, std::fabs(t[i])); l1 += std::fabs(t[i]); }
+        int x = 0;
+        (void)std::frexp(cmax, &x);             // cmax < 2^x
+        e[l] = 12 - x;
+        a.mp[l] = std::ldexp(1.f, e[l]);
+        if (l == 0) (void)std::frexp(l1 * a.mp[0], &eq);   // l1 * mp < 2^eq
+    }
+    eq = std::max(eq, 0);
+    a.mp[2] = 1.f;
+    a.mq = std::ldexp(1.f, -eq);
+    a.mu[0] = std::ldexp(1.f, -(e[0] + e[1] - eq));
+    return launch(g_fir_mfma_table[std::max(rV, rH) <= 8 ? 0 : 1][in_dt][out_dt], a, s);
 }
 
 // Workgroup shape of the single-image matrix-core kernel: TUCU_MFMA_ROWS
@@ -407,6 +440,15 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
     centre_index(lobeH, rH, nH & 1, a.ch);
     if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
     if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
+    if (fir_use_mfma()) {
+        // full taps: c[k] on the upper / left side, sgn * c[k] on the other (FIRFilter.h:66-176)
+        float vt[TMAX] = {0.f}, ht[TMAX] = {0.f};
+        for (int k = 0; k <= 16; ++k) {
+            vt[16 - k] = a.cv[0][k]; vt[16 + k] = k ? a.sgnv * a.cv[0][k] : a.cv[0][0];
+            ht[16 - k] = a.ch[k];    ht[16 + k] = k ? a.sgnh * a.ch[k] : a.ch[0];
+        }
+        return launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, static_cast<hipStream_t>(stream));
+    }
     const KernelEntry& k = g_fir_table[fir_class(rV)][fir_class(rH)][in->dtype][out->dtype];
     return launch(k, a, static_cast<hipStream_t>(stream));
 }
@@ -440,6 +482,12 @@ int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int 
     for (int i = 0; i < nH; ++i) a.th[RH - rH + i] = tapsH[i];
     if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
     if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
+    if (fir_use_mfma()) {
+        float vt[TMAX] = {0.f}, ht[TMAX] = {0.f};
+        for (int i = 0; i < nV; ++i) vt[16 - rV + i] = tapsV[i];
+        for (int i = 0; i < nH; ++i) ht[16 - rH + i] = tapsH[i];
+        return launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, static_cast<hipStream_t>(stream));
+    }
     const KernelEntry& k = g_taps_table[cls(rV)][cls(rH)][in->dtype][out->dtype];
     return launch(k, a, static_cast<hipStream_t>(stream));
 }

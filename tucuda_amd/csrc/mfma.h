@@ -71,24 +71,33 @@ struct MfmaTabs {
     v4f16 run[2 * 3 * MT4];
 };
 
-template <class A>
+// MODE_GAUSS: the three lobes of the Gaussian (centre-indexed cv[l]).
+// MODE_TAPS (one output of FIRFilter2 or of general taps): table 0 holds the
+// vertical taps, table 1 the horizontal ones, both as full taps with tap o
+// at index o + 16 of tv / th.
+template <int MODE, class A>
 __device__ __forceinline__ float mfma_tap(const A& a, int l, int o)
 {
     const int k = o < 0 ? -o : o;
     if (k > 16) return 0.f;
-    const float c = a.cv[l][k];
-    return l != 1 ? c : o == 0 ? 0.f : o < 0 ? c : -c;
+    if constexpr (MODE == MODE_TAPS) {
+        return l == 0 ? a.tv[o + 16] : l == 1 ? a.th[o + 16] : 0.f;
+    } else {
+        const float c = a.cv[l][k];
+        return l != 1 ? c : o == 0 ? 0.f : o < 0 ? c : -c;
+    }
 }
 
-template <class A>
+template <int MODE = MODE_GAUSS, class A>
 __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
 {
-    for (int e = threadIdx.x; e < 3 * MT4; e += blockDim.x) {
+    constexpr int NL = MODE == MODE_TAPS ? 2 : 3;
+    for (int e = threadIdx.x; e < NL * MT4; e += blockDim.x) {
         const int l = e / MT4, t = e - l * MT4;
         v4f16 hi, lo;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float c = mfma_tap(a, l, t + i - 32) * a.mp[l];
+            const float c = mfma_tap<MODE>(a, l, t + i - 32) * a.mp[l];
             const _Float16 h = _Float16(c);
             hi[i] = h;
             lo[i] = _Float16(c - float(h));
@@ -222,10 +231,11 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
 //     holds columns 4 g .. 4 g + 3 of each, window columns 4 g + j and
 //     16 + 4 g + j); A = taps of offsets (window column) - RC - m.
 //   horizontal, K48 second slice: B = field blocks o + 1 (weight 0), o + 2.
-template <int RC, class IN, class OUT, class A, class F>
+template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
 __device__ __forceinline__ void
 mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
 {
+    constexpr bool ONE = MODE == MODE_TAPS;     // one field (table 0), one output (table 1 across)
     using G = MGeo<RC>;
     constexpr int NVB = G::NVB, NLD = G::NLD;
     constexpr bool XLO = sizeof(IN) == 4;       // f16 / bf16 / u8 inputs are exact in one f16
@@ -266,7 +276,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // x0 - RC + 16 b + 4 g .. +3 (the accumulator layout), split in f16.
     // One lobe at a time, the chains of the field blocks interleaved.
     vfield fh[3], fl[3];
-    const unsigned fneed[3] = {mask & 0x0bu, mask & 0x14u, mask & 0x20u};
+    const unsigned fneed[3] = {ONE ? 1u : mask & 0x0bu, ONE ? 0u : mask & 0x14u, ONE ? 0u : mask & 0x20u};
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
         if (!fneed[l]) continue;
@@ -321,10 +331,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // ------------------------------------------------- horizontal pass
     const bool fast = y0 >= a.fy0 && y0 + 16 <= a.fy1 && x0 >= a.fx0 && x0 + MSTRIP_W <= a.fx1;
     const int y = y0 + m;
-    constexpr int HL[6] = {0, 1, 0, 2, 1, 0};   // horizontal lobe of output k
-    constexpr int VL[6] = {0, 0, 1, 0, 1, 2};   // vertical field of output k
+    constexpr int NOUT  = ONE ? 1 : 6;
+    constexpr int HL[6] = {ONE ? 1 : 0, 1, 0, 2, 1, 0};   // horizontal table of output k
+    constexpr int VL[6] = {0, 0, 1, 0, 1, 2};             // vertical field of output k
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < NOUT; ++k) {
         if (!(mask >> k & 1u)) continue;
         const int h = HL[k], v = VL[k];
         const int t1 = 32 + 4 * g - RC - m;
@@ -393,26 +404,26 @@ __device__ __forceinline__ bool mfma_strip_pos(const A& a, int tile, int& y0, in
 }
 
 // One image, one workgroup per tile.
-template <int RC, int WY, class IN, class OUT>
+template <int RC, int WY, class IN, class OUT, int MODE = MODE_GAUSS>
 __global__ void __launch_bounds__(64 * MWAVES)
 sepfir_mfma_kernel(const SepArgs a)
 {
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
-    mfma_build_tabs(a, T);
+    mfma_build_tabs<MODE>(a, T);
     __syncthreads();
     int y0, x0;
     if (!mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0)) return;
     MfmaRaw raw;
     mfma_load<RC, IN>(a, y0, x0, raw);
-    mfma_compute<RC, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH,
-                              [] {});
+    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T,
+                                    reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH, [] {});
 }
 
-template <int RC, int WY, class IN, class OUT>
+template <int RC, int WY, class IN, class OUT, int MODE = MODE_GAUSS>
 hipError_t launch_sepfir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((sepfir_mfma_kernel<RC, WY, IN, OUT>), dim3(nblocks), dim3(64 * MWAVES), 0, s, a);
+    hipLaunchKernelGGL((sepfir_mfma_kernel<RC, WY, IN, OUT, MODE>), dim3(nblocks), dim3(64 * MWAVES), 0, s, a);
     return hipGetLastError();
 }
 
@@ -525,5 +536,6 @@ constexpr int NMWY = 3;
 constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
 constexpr int MFMA_WY_DEFAULT = 1;          // index: 2 x 2 strips, 32 x 128 pixels
 extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
+extern KernelEntry g_fir_mfma_table[2][4][4];       // MODE_TAPS, default shape: [radius class][in][out]
 
 }   // namespace tucu
