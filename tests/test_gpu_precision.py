@@ -19,7 +19,6 @@ from tucuda_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RESULTS = {}
 
 
@@ -30,9 +29,11 @@ def _errors(got: np.ndarray, ref: np.ndarray):
 
 
 def _record(name, errs):
+    """Keeps the error; with TUCU_RESULTS_DIR set, also writes every error so
+    far to precision_errors.json there."""
     RESULTS[name] = errs
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("TUCU_RESULTS_DIR")
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "precision_errors.json"), "w") as f:
             json.dump(RESULTS, f, indent=1)
 
