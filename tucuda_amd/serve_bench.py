@@ -12,7 +12,8 @@ arrival time, so it includes any lag of the submitting loop.  Outputs are
 released as soon as they are polled.
 
 A capacity run then submits ``--capacity-requests`` requests at once (the
-pool's admission throttles them) and reports the served throughput next to
+pool's admission throttles them) and reports the served throughput, latency
+and batch sizes under that saturation next to
 the plain per-image forms on the same requests, float32 and bfloat16 images:
 one fused launch per request, and one launch per requested output (the
 method-per-output calls of FIRGaussianConvolver2); all launches captured in
@@ -187,31 +188,41 @@ def run_stream(srv, images, rate: float, duration: float, seed: int, check_every
 
 
 def run_capacity(srv, images, n: int, seed: int):
-    """n requests submitted at once; served throughput in Gpix/s of input."""
+    """n requests submitted at once (each stamped with its own submission
+    time); served throughput in Gpix/s of input, and the latency and batch
+    sizes under that saturation."""
     reqs = make_requests(n, len(images), seed)
     pix = sum(images[ii].shape[0] * images[ii].shape[1] for ii, _, _ in reqs)
     pending = set()
+    lat = []
+    st0 = srv.stats()
+
+    def drain():
+        for c in srv.poll():
+            if c["status"] != 0:
+                raise RuntimeError(f"request {c['id']} failed with status {c['status']}")
+            pending.discard(c["id"])
+            lat.append(c["completed"] - c["arrival"])
+            srv.release(c["id"])
+
     t0 = srv.now()
     for ii, sigma, mask in reqs:
-        pending.add(srv.submit(images[ii], sigma, mask, arrival=t0))
-        for c in srv.poll():
-            if c["status"] != 0:
-                raise RuntimeError(f"request {c['id']} failed with status {c['status']}")
-            pending.discard(c["id"])
-            srv.release(c["id"])
+        pending.add(srv.submit(images[ii], sigma, mask))
+        drain()
     deadline = time.monotonic() + 120.0
     while pending and time.monotonic() < deadline:
-        for c in srv.poll():
-            if c["status"] != 0:
-                raise RuntimeError(f"request {c['id']} failed with status {c['status']}")
-            pending.discard(c["id"])
-            srv.release(c["id"])
+        drain()
         time.sleep(5e-5)
     if pending:
         raise RuntimeError(f"{len(pending)} requests still pending after 120 s")
     dt = srv.now() - t0
+    st1 = srv.stats()
+    batches = st1["batches"] - st0["batches"]
     return reqs, pix, {"requests": n, "seconds": round(dt, 4), "served_req_s": round(n / dt, 1),
-                       "served_gpix_s": round(pix / dt / 1e9, 2)}
+                       "served_gpix_s": round(pix / dt / 1e9, 2),
+                       "p50_ms": round(_pct(lat, 50) * 1e3, 3), "p99_ms": round(_pct(lat, 99) * 1e3, 3),
+                       "batches": batches, "mean_batch": round(n / max(batches, 1), 1),
+                       "max_batch": st1["max_batch"], "pool_peak_gib": round(st1["pool_peak"] / 2**30, 3)}
 
 
 def plain_per_image(images, reqs, dtype, reps: int = 3, per_method: bool = False):
