@@ -190,7 +190,7 @@ int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
     }
     b->rclass = rmax_all;
     b->bucket = work > 0 ? int(std::log2(double(work))) : 0;
-    if (hipMalloc(&b->d_counter, 128) != hipSuccess) {
+    if (hipMalloc(&b->d_counter, QUEUE_BYTES) != hipSuccess) {
         (void)hipGetLastError();
         delete b;
         return fail(TUCU_ERR_BAD_ALLOC, "device allocation for the batch counter failed");

@@ -427,7 +427,7 @@ hipError_t launch_sepfir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
     return hipGetLastError();
 }
 
-// Matrix-core variant of batch_kernel: the same tile scheduling; the tap
+// Matrix-core variant of batch_kernel: the same tile queue; the tap
 // tables in LDS are rebuilt when the job changes (tiles of a job are
 // contiguous in the list).
 template <int WY, class IN, class OUT>
@@ -441,31 +441,35 @@ batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict
     ConstArgs* const jobs = (ConstArgs*)(uintptr_t)jobs_;
     float* const stage = reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH;
     int cur = -1;
-    int ti = int(blockIdx.x);
-    while (ti < ntiles) {
+    TileQueue q;
+    int ti = q.first(ntiles);
+    for (;;) {
         if (threadIdx.x == 0)          // the answer is read after this tile
-            next = int(gridDim.x) + __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const BatchTile e = tiles[ti];
-        ConstArgs& a = jobs[e.job];
-        if (int(e.job) != cur) {       // uniform over the workgroup
-            mfma_build_tabs(a, T);
-            cur = int(e.job);
-            __syncthreads();
-        }
-        int y0, x0;
-        if (mfma_strip_pos<WY>(a, int(e.tile), y0, x0)) {
-            MfmaRaw raw;
-            if (a.rt <= 8) {
-                mfma_load<8, IN>(a, y0, x0, raw);
-                mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
-            } else {
-                mfma_load<16, IN>(a, y0, x0, raw);
-                mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
+            next = q.grab(counter, ntiles);
+        if (ti >= 0) {
+            const BatchTile e = tiles[ti];
+            ConstArgs& a = jobs[e.job];
+            if (int(e.job) != cur) {   // uniform over the workgroup
+                mfma_build_tabs(a, T);
+                cur = int(e.job);
+                __syncthreads();
+            }
+            int y0, x0;
+            if (mfma_strip_pos<WY>(a, int(e.tile), y0, x0)) {
+                MfmaRaw raw;
+                if (a.rt <= 8) {
+                    mfma_load<8, IN>(a, y0, x0, raw);
+                    mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
+                } else {
+                    mfma_load<16, IN>(a, y0, x0, raw);
+                    mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
+                }
             }
         }
         __syncthreads();               // T reuse; `next` was written before this barrier
         ti = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();               // all have read `next` before thread 0 writes it again
+        if (ti >= ntiles) break;
     }
 }
 
@@ -482,33 +486,37 @@ batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __res
     __shared__ v4f32 stage4[16 * MOPITCH / 4];
     ConstArgs* const jobs = (ConstArgs*)(uintptr_t)jobs_;
     int cur = -1;
-    int ti = int(blockIdx.x);
-    while (ti < ntiles) {
+    TileQueue q;
+    int ti = q.first(ntiles);
+    for (;;) {
         int nxt = 0;
         if (threadIdx.x == 0)          // the answer is read after this strip
-            nxt = int(gridDim.x) + __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const BatchTile e = tiles[ti];
-        ConstArgs& a = jobs[e.job];
-        if (int(e.job) != cur) {       // LDS accesses of one wave run in order
-            mfma_build_tabs(a, T);
-            cur = int(e.job);
-            __builtin_amdgcn_wave_barrier();
-        }
-        const int tx = int(e.tile) / a.tiles_y;
-        const int y0 = a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y);
-        const int x0 = a.cx0 + MSTRIP_W * tx;
-        if (y0 < a.cy1 && x0 < a.cx1) {
-            MfmaRaw raw;
-            if (a.rt <= 8) {
-                mfma_load<8, IN>(a, y0, x0, raw);
-                mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
-            } else {
-                mfma_load<16, IN>(a, y0, x0, raw);
-                mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+            nxt = q.grab(counter, ntiles);
+        if (ti >= 0) {
+            const BatchTile e = tiles[ti];
+            ConstArgs& a = jobs[e.job];
+            if (int(e.job) != cur) {   // LDS accesses of one wave run in order
+                mfma_build_tabs(a, T);
+                cur = int(e.job);
+                __builtin_amdgcn_wave_barrier();
+            }
+            const int tx = int(e.tile) / a.tiles_y;
+            const int y0 = a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y);
+            const int x0 = a.cx0 + MSTRIP_W * tx;
+            if (y0 < a.cy1 && x0 < a.cx1) {
+                MfmaRaw raw;
+                if (a.rt <= 8) {
+                    mfma_load<8, IN>(a, y0, x0, raw);
+                    mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+                } else {
+                    mfma_load<16, IN>(a, y0, x0, raw);
+                    mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
         ti = __builtin_amdgcn_readfirstlane(nxt);
+        if (ti >= ntiles) break;
     }
 }
 
@@ -516,7 +524,7 @@ template <class IN, class OUT>
 hipError_t launch_batch_mfma_wave(const SepArgs* jobs, const BatchTile* tiles, int ntiles, int* counter,
                                   int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(reset_counter<1>, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(reset_queue<>, dim3(1), dim3(64), 0, s, counter, grid);
     hipLaunchKernelGGL((batch_mfma_wave_kernel<IN, OUT>), dim3(grid), dim3(64), 0, s, jobs, tiles, ntiles, counter);
     return hipGetLastError();
 }
@@ -525,7 +533,7 @@ template <int WY, class IN, class OUT>
 hipError_t launch_batch_mfma(const SepArgs* jobs, const BatchTile* tiles, int ntiles, int* counter,
                              int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(reset_counter<1>, dim3(1), dim3(64), 0, s, counter);
+    hipLaunchKernelGGL(reset_queue<>, dim3(1), dim3(64), 0, s, counter, grid);
     hipLaunchKernelGGL((batch_mfma_kernel<WY, IN, OUT>), dim3(grid), dim3(64 * MWAVES), 0, s, jobs, tiles, ntiles, counter);
     return hipGetLastError();
 }

@@ -380,7 +380,7 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, size_t(s->cfg.max_batch_jobs) * sizeof(SepArgs));
         if (e == hipSuccess) e = hipMalloc(&sl.d_tiles, size_t(s->cfg.max_tiles) * sizeof(BatchTile));
-        if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 128);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_counter, QUEUE_BYTES);
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_jobs, size_t(s->cfg.max_batch_jobs) * sizeof(SepArgs));
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_tiles, size_t(s->cfg.max_tiles) * sizeof(BatchTile));
         if (e != hipSuccess) {
