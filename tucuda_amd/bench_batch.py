@@ -16,6 +16,11 @@ bfloat16 storage.  Compared forms, all on the same resident buffers:
 * ``batched``: one persistent launch over every image, tile height picked by
   the autotuner, replayed from a captured HIP graph.
 
+Images and outputs have the layout of ``cu::Array2<T>`` (rows padded to 256
+bytes, ``tucuda_amd.array2``), the reference's device image; ``--packed``
+uses rows packed at their width instead (most widths then start rows off
+cache-line boundaries).
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -49,6 +54,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--dtypes", default="float32,bfloat16")
+    ap.add_argument("--packed", action="store_true", help="rows packed at their width instead of a 256-byte pitch")
     args = ap.parse_args(argv)
 
     import torch
@@ -59,7 +65,13 @@ def main(argv=None):
     sizes = make_sizes(args.images, args.seed)
     total_px = sum(h * w for h, w, _ in sizes)
     result = {"workload": f"{args.images} images, log-uniform [240,2160], sigma U[0.8,4], radius ceil(4 sigma), 6 outputs",
-              "images": args.images, "megapixels": round(total_px / 1e6, 2), "forms": {}}
+              "images": args.images, "megapixels": round(total_px / 1e6, 2),
+              "layout": "rows packed" if args.packed else "256-byte row pitch (cu::Array2)", "forms": {}}
+
+    def image(h, w, dt):
+        if args.packed:
+            return torch.empty((h, w), dtype=dt, device=dev)
+        return tucuda_amd.array2(h, w, dt, dev)
 
     def timed(fn, reps):
         fn()
@@ -91,9 +103,10 @@ def main(argv=None):
         dt = getattr(torch, dname)
         ins, outs = [], []
         for i, (h, w, _) in enumerate(sizes):
-            img = synth.edge_image(1000 + i, h, w)
-            ins.append(torch.from_numpy(img).to(dev).to(dt))
-            outs.append([torch.empty((h, w), dtype=dt, device=dev) for _ in range(6)])
+            x = image(h, w, dt)
+            x.copy_(torch.from_numpy(synth.edge_image(1000 + i, h, w)))
+            ins.append(x)
+            outs.append([image(h, w, dt) for _ in range(6)])
         gs = [tucuda_amd.FIRGaussianConvolver2(s, rule=tucuda_amd.TUCU_RULE_CEIL4SIGMA) for _, _, s in sizes]
 
         def plain():

@@ -150,6 +150,8 @@ int check_image(const tucu_image* im, const char* name, bool need_data)
     if (dtype_size(im->dtype) == 0) return fail(TUCU_ERR_INVALID, std::string(name) + ": unknown dtype");
     if (im->nrow < 0 || im->ncol < 0) return fail(TUCU_ERR_INVALID, std::string(name) + ": negative size");
     if (need_data && !im->data) return fail(TUCU_ERR_INVALID, std::string(name) + ": null data");
+    if (reinterpret_cast<uintptr_t>(im->data) % dtype_size(im->dtype) != 0)
+        return fail(TUCU_ERR_INVALID, std::string(name) + ": data not aligned to its element size");
     if (im->stride < im->ncol) return fail(TUCU_ERR_INVALID, std::string(name) + ": stride shorter than a row");
     return TUCU_OK;
 }

@@ -69,6 +69,7 @@ template <int RC> struct MGeo {
 // with 8-byte LDS reads and needs no conversion.
 struct MfmaTabs {
     v4f16 run[2 * 3 * MT4];
+    float tap[3][MT4 + 4];     // staging: tap o of lobe l times mp[l] at [l][o + 32]
 };
 
 // MODE_GAUSS: the three lobes of the Gaussian (centre-indexed cv[l]).
@@ -88,16 +89,26 @@ __device__ __forceinline__ float mfma_tap(const A& a, int l, int o)
     }
 }
 
+// Every thread of the workgroup calls it (it holds a barrier); the caller
+// puts another barrier between it and the first use of T.  The taps are
+// staged in LDS first, one kernel-argument load per thread, so the build
+// costs one memory round trip rather than one per tap of a run.
 template <int MODE = MODE_GAUSS, class A>
 __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
 {
     constexpr int NL = MODE == MODE_TAPS ? 2 : 3;
+    constexpr int NT4 = MT4 + 4;
+    for (int e = threadIdx.x; e < NL * NT4; e += blockDim.x) {
+        const int l = e / NT4, t = e - l * NT4;
+        T.tap[l][t] = mfma_tap<MODE>(a, l, t - 32) * a.mp[l];
+    }
+    __syncthreads();
     for (int e = threadIdx.x; e < NL * MT4; e += blockDim.x) {
         const int l = e / MT4, t = e - l * MT4;
         v4f16 hi, lo;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float c = mfma_tap<MODE>(a, l, t + i - 32) * a.mp[l];
+            const float c = T.tap[l][t + i];
             const _Float16 h = _Float16(c);
             hi[i] = h;
             lo[i] = _Float16(c - float(h));
@@ -121,20 +132,20 @@ __device__ __forceinline__ float wave_max(float v)
 
 // Input values of field block b for lane (g, m): column x0 - RC + 16 b + m,
 // rows y0 - RC + 8 g + j (j < 8) and, for K = 48, y0 - RC + 32 + 4 g + j
-// (j < 4).  Pixels outside the image read as 0 (buffer range check).
-template <int RC, class IN, class A>
+// (j < 4).  INSIDE: every pixel of the strip's window lies in the image
+// (row offsets as scalar offsets); otherwise pixels outside the image read
+// as 0 (buffer range check).
+template <int RC, bool INSIDE, class IN, class A>
 __device__ __forceinline__ void
-mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, int g, int m,
-                bool rows_ok, float* v)
+mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, int g, int m, float* v)
 {
     constexpr int NLD = MGeo<RC>::NLD;
     const int es = int(sizeof(IN));
-    const int xb = x0 - RC + 16 * b;
-    const int col = xb + m;
+    const int col = x0 - RC + 16 * b + m;
     const int sb = a.in_stride_b;
     const int r0 = y0 - RC + 8 * g;            // first row of entries 0 .. 7
     const int r1 = y0 - RC + 32 + 4 * g;       // first row of entries 8 .. 11
-    if (rows_ok && xb >= 0 && xb + 16 <= a.ncol) {          // uniform: every pixel inside
+    if constexpr (INSIDE) {
         const int v0 = r0 * sb + col * es, v1 = r1 * sb + col * es;
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = Px<IN>::ld(rsrc, v0, j * sb);
@@ -207,11 +218,21 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
     const int g = lane >> 4, m = lane & 15;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
-    const bool rows_ok = y0 - RC >= 0 && y0 - RC + G::KV <= a.nrow;
+    // One test for the whole window (y0, x0 are wave-uniform): with a test
+    // per block, the compiler makes each block's edge path wait for the
+    // loads of the blocks before it.
+    const bool inside = y0 - RC >= 0 && y0 - RC + G::KV <= a.nrow &&
+                        x0 - RC >= 0 && x0 - RC + 16 * G::NVB <= a.ncol;
+    if (inside) {
 #pragma unroll
-    for (int b = 0; b < G::NVB; ++b) {
-        mfma_load_block<RC, IN>(a, rsrc, y0, x0, b, g, m, rows_ok, raw[b]);
-        if constexpr (!G::K48) raw[b][8] = raw[b][9] = raw[b][10] = raw[b][11] = 0.f;
+        for (int b = 0; b < G::NVB; ++b) mfma_load_block<RC, true, IN>(a, rsrc, y0, x0, b, g, m, raw[b]);
+    } else {
+#pragma unroll
+        for (int b = 0; b < G::NVB; ++b) mfma_load_block<RC, false, IN>(a, rsrc, y0, x0, b, g, m, raw[b]);
+    }
+    if constexpr (!G::K48) {
+#pragma unroll
+        for (int b = 0; b < G::NVB; ++b) raw[b][8] = raw[b][9] = raw[b][10] = raw[b][11] = 0.f;
     }
 }
 
@@ -398,24 +419,29 @@ __device__ __forceinline__ bool mfma_strip_pos(const A& a, int tile, int& y0, in
     const int tx = tile / a.tiles_y;
     const int ty = tile - tx * a.tiles_y;
     const int w  = int(threadIdx.x) >> 6;
-    y0 = a.cy0 + ty * (16 * WY) + 16 * (w / WX);
-    x0 = a.cx0 + tx * (MSTRIP_W * WX) + MSTRIP_W * (w % WX);
+    // wave-uniform, and held in scalar registers so that every test on the
+    // strip's position is a scalar branch: a divergent if / else around the
+    // loads makes the compiler wait for the loads already in flight before
+    // the other side's address arithmetic
+    y0 = __builtin_amdgcn_readfirstlane(a.cy0 + ty * (16 * WY) + 16 * (w / WX));
+    x0 = __builtin_amdgcn_readfirstlane(a.cx0 + tx * (MSTRIP_W * WX) + MSTRIP_W * (w % WX));
     return y0 < a.cy1 && x0 < a.cx1;
 }
 
 // One image, one workgroup per tile.
 template <int RC, int WY, class IN, class OUT, int MODE = MODE_GAUSS>
-__global__ void __launch_bounds__(64 * MWAVES)
+__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))
 sepfir_mfma_kernel(const SepArgs a)
 {
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    int y0, x0;
+    const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
+    MfmaRaw raw;
+    if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
     mfma_build_tabs<MODE>(a, T);
     __syncthreads();
-    int y0, x0;
-    if (!mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0)) return;
-    MfmaRaw raw;
-    mfma_load<RC, IN>(a, y0, x0, raw);
+    if (!live) return;
     mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T,
                                     reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH, [] {});
 }
@@ -431,7 +457,7 @@ hipError_t launch_sepfir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
 // tables in LDS are rebuilt when the job changes (tiles of a job are
 // contiguous in the list).
 template <int WY, class IN, class OUT>
-__global__ void __launch_bounds__(64 * MWAVES)
+__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))
 batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict__ tiles, int ntiles,
                   int* __restrict__ counter)
 {
@@ -478,7 +504,7 @@ batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict
 // the waves of a SIMD drift apart and their loads, matrix work and stores
 // overlap.  Each wave keeps its own tap tables.
 template <class IN, class OUT>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict__ tiles, int ntiles,
                        int* __restrict__ counter)
 {
@@ -495,22 +521,23 @@ batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __res
         if (ti >= 0) {
             const BatchTile e = tiles[ti];
             ConstArgs& a = jobs[e.job];
-            if (int(e.job) != cur) {   // LDS accesses of one wave run in order
+            if (int(e.job) != cur) {   // one wave: its barrier orders its LDS accesses
                 mfma_build_tabs(a, T);
                 cur = int(e.job);
-                __builtin_amdgcn_wave_barrier();
+                __syncthreads();
             }
             const int tx = int(e.tile) / a.tiles_y;
-            const int y0 = a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y);
-            const int x0 = a.cx0 + MSTRIP_W * tx;
+            const int y0 = __builtin_amdgcn_readfirstlane(a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y));
+            const int x0 = __builtin_amdgcn_readfirstlane(a.cx0 + MSTRIP_W * tx);
             if (y0 < a.cy1 && x0 < a.cx1) {
                 MfmaRaw raw;
+                float* const st = reinterpret_cast<float*>(stage4);
                 if (a.rt <= 8) {
                     mfma_load<8, IN>(a, y0, x0, raw);
-                    mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+                    mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, st, [] {});
                 } else {
                     mfma_load<16, IN>(a, y0, x0, raw);
-                    mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, reinterpret_cast<float*>(stage4), [] {});
+                    mfma_compute<16, IN, OUT>(a, y0, x0, raw, T, st, [] {});
                 }
             }
         }

@@ -48,6 +48,20 @@ def image_desc(t: torch.Tensor, with_extent: bool = True) -> TucuImage:
                      _DTYPES[t.dtype], 0)
 
 
+def array2(nrow: int, ncol: int, dtype=torch.float32, device="cuda") -> torch.Tensor:
+    """A zero-filled device image laid out like ``cu::Array2<T>``: rows
+    padded to a multiple of 256 bytes (allocator.h:68, Array++.h:142),
+    returned as the ``nrow x ncol`` view.  Rows that start on cache-line
+    boundaries let the kernels write whole lines; with packed rows of an
+    arbitrary width, the partial lines at tile edges cost the write stream
+    much of its rate."""
+    if dtype not in _DTYPES:
+        raise TypeError(f"unsupported element type {dtype}")
+    es = torch.empty((), dtype=dtype).element_size()
+    stride = int(_capi.load().tucu_row_stride(int(ncol), es))
+    return torch.zeros((int(nrow), max(stride, 1)), dtype=dtype, device=device)[:, :int(ncol)]
+
+
 def _stream_handle(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
