@@ -220,11 +220,11 @@ typedef struct {
     int32_t in_dtype;         /* element type of every request's input */
     int32_t out_dtype;        /* element type of the outputs */
     int32_t rule;             /* TUCU_RULE_* for the lobes of each sigma */
-    int32_t tile_rows;        /* batched-kernel tile height: 4, 8 or 16 (0: 8) */
+    int32_t tile_rows;        /* batched-kernel tile height (0: the library default; as tucu_batch_set_tile_rows) */
     int32_t max_batch_jobs;   /* requests per launch (0: 1024) */
     int32_t max_inflight;     /* launches in flight at once (0: 2) */
     int32_t max_tiles;        /* tiles per launch (0: 2^21) */
-    int32_t reserved;
+    int32_t max_batch_mpix;   /* input megapixels per launch (0: 64); a launch holds at least one request */
 } tucu_server_config;
 
 typedef struct {
@@ -255,6 +255,10 @@ TUCU_API int    tucu_server_destroy(tucu_server server);
 /* Seconds since the server was created (the clock of all timestamps). */
 TUCU_API double tucu_server_now(tucu_server server);
 TUCU_API int    tucu_server_submit(tucu_server server, const tucu_request* req, uint64_t* id);
+/* Submits n requests at once, in order (one lock, one wake-up of the
+ * scheduler): all are checked first and none is queued if one is invalid;
+ * ids (n entries, may be null) receives their ids. */
+TUCU_API int    tucu_server_submit_many(tucu_server server, const tucu_request* reqs, int n, uint64_t* ids);
 /* Moves up to max completions into out; *n = number moved. */
 TUCU_API int    tucu_server_poll(tucu_server server, tucu_completion* out, int max, int* n);
 TUCU_API int    tucu_server_release(tucu_server server, uint64_t id);

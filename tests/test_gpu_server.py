@@ -208,3 +208,32 @@ def test_server_errors(dev):
         srv.release(rid)
         with pytest.raises(tucuda_amd.TucuError):
             srv.release(rid)                                  # already released
+
+
+def test_submit_many(dev):
+    """Requests submitted in one call run like ones submitted one by one
+    (same outputs as their single-image launches, ids in order), and a call
+    holding one invalid request queues none of them."""
+    import torch
+    import tucuda_amd
+    from tucuda_amd.serve import Server
+    reqs = _requests(dev, 10, 57, torch.bfloat16, sizes=(64, 300))
+    with Server(1 << 28) as srv:
+        bad = [(x, s, m) for x, s, m, _ in reqs[:3]] + [(reqs[3][0], 2.0, 0)]
+        with pytest.raises(tucuda_amd.TucuError):
+            srv.submit_many(bad)                              # the last one asks for no output
+        assert srv.stats()["requests_launched"] == 0
+        ids = srv.submit_many([(x, s, m) for x, s, m, _ in reqs])
+        assert ids == sorted(ids) and len(set(ids)) == len(reqs)
+        got = _wait(srv, ids)
+        assert srv.stats()["max_batch"] > 1
+        for rid, (x, sigma, mask, _) in zip(ids, reqs):
+            c = got[rid]
+            assert c["status"] == 0
+            outs = _outputs(srv, c, x, mask)
+            ref = _single(x, sigma, mask)
+            for k in outs:
+                _same(outs[k], ref[k], f"req {rid} out {k}")
+            srv.release(rid)
+        assert srv.stats()["pool_used"] == 0
+        assert srv.submit_many([]) == []

@@ -31,7 +31,7 @@ EXPORTS = (
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
     "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
     "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
-    "tucu_server_poll", "tucu_server_release", "tucu_server_stats",
+    "tucu_server_submit_many", "tucu_server_poll", "tucu_server_release", "tucu_server_stats",
 )
 
 
@@ -71,7 +71,7 @@ class TucuServerConfig(ctypes.Structure):
         ("max_batch_jobs", ctypes.c_int32),
         ("max_inflight", ctypes.c_int32),
         ("max_tiles", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("max_batch_mpix", ctypes.c_int32),
     ]
 
 
@@ -184,6 +184,8 @@ def load() -> ctypes.CDLL:
     L.tucu_server_now.argtypes = [vp]
     L.tucu_server_now.restype = ctypes.c_double
     L.tucu_server_submit.argtypes = [vp, ctypes.POINTER(TucuRequest), ctypes.POINTER(ctypes.c_uint64)]
+    L.tucu_server_submit_many.argtypes = [vp, ctypes.POINTER(TucuRequest), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_uint64)]
     L.tucu_server_poll.argtypes = [vp, ctypes.POINTER(TucuCompletion), ctypes.c_int, ip]
     L.tucu_server_release.argtypes = [vp, ctypes.c_uint64]
     L.tucu_server_stats.argtypes = [vp, ctypes.POINTER(TucuServerCounters)]

@@ -242,11 +242,18 @@ bool admit_and_launch(tucu_server_s& s, Slot& sl)
     std::vector<SepArgs> jobs;
     std::vector<Request*> reqs;
     size_t tiles_est = 0;
+    int64_t pixels = 0;
+    const int64_t max_pixels = int64_t(s.cfg.max_batch_mpix) << 20;
     while (!s.waiting.empty() && int(jobs.size()) < s.cfg.max_batch_jobs) {
         Request* r = s.waiting.front();
         // tiles at the largest radius (16) and one extra row and column, an upper bound
         const size_t est = size_t((r->req.in.nrow + e.th - 1) / e.th + 1) * size_t(r->req.in.ncol / tile_w + 2);
         if (!jobs.empty() && tiles_est + est > size_t(s.cfg.max_tiles)) break;
+        // launches of bounded work, so that several are in flight and
+        // completions (and the pool memory they free) come steadily
+        const int64_t px = int64_t(r->req.in.nrow) * r->req.in.ncol;
+        if (!jobs.empty() && pixels + px > max_pixels) break;
+        pixels += px;
         if (!allocate_outputs(s, r)) break;                   // admission by free HBM: FIFO, head blocks
         const Lobes& l = lobes_for(s, r->req.sigma);
         const float* lob[3] = {&l[0], &l[17], &l[34]};
@@ -366,6 +373,7 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
     if (s->cfg.max_batch_jobs <= 0) s->cfg.max_batch_jobs = 1024;
     if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
     if (s->cfg.max_tiles <= 0) s->cfg.max_tiles = 1 << 21;
+    if (s->cfg.max_batch_mpix <= 0) s->cfg.max_batch_mpix = 64;
     s->t0 = std::chrono::steady_clock::now();
     (void)hipGetDevice(&s->device);
     if (hipMalloc(&s->arena, s->cfg.pool_bytes) != hipSuccess) {
@@ -413,28 +421,54 @@ double tucu_server_now(tucu_server server)
     return server ? server->now() : 0.0;
 }
 
+namespace {
+
+int check_request(const tucu_server_s& s, const tucu_request& req)
+{
+    if (req.in.dtype != s.cfg.in_dtype) return fail(TUCU_ERR_INVALID, "input dtype differs from the server's");
+    if (req.mask == 0 || (req.mask & ~unsigned(TUCU_ALL6))) return fail(TUCU_ERR_INVALID, "bad output mask");
+    if (!req.in.data || req.in.nrow <= 0 || req.in.ncol <= 0) return fail(TUCU_ERR_INVALID, "bad input image");
+    const size_t es = dtype_size(s.cfg.out_dtype);
+    const size_t row = round_up(size_t(req.in.ncol) * es, 128);
+    if (size_t(__builtin_popcount(req.mask)) * round_up(row * size_t(req.in.nrow), ALIGN) > s.cfg.pool_bytes)
+        return fail(TUCU_ERR_BAD_ALLOC, "request larger than the server's output pool");
+    return TUCU_OK;
+}
+
+}   // namespace
+
 int tucu_server_submit(tucu_server server, const tucu_request* req, uint64_t* id)
 {
-    if (!server || !req) return fail(TUCU_ERR_INVALID, "null argument");
-    if (req->in.dtype != server->cfg.in_dtype) return fail(TUCU_ERR_INVALID, "input dtype differs from the server's");
-    if (req->mask == 0 || (req->mask & ~unsigned(TUCU_ALL6))) return fail(TUCU_ERR_INVALID, "bad output mask");
-    if (!req->in.data || req->in.nrow <= 0 || req->in.ncol <= 0) return fail(TUCU_ERR_INVALID, "bad input image");
-    const size_t es = dtype_size(server->cfg.out_dtype);
-    const size_t row = round_up(size_t(req->in.ncol) * es, 128);
-    if (size_t(__builtin_popcount(req->mask)) * round_up(row * size_t(req->in.nrow), ALIGN) > server->cfg.pool_bytes)
-        return fail(TUCU_ERR_BAD_ALLOC, "request larger than the server's output pool");
-    auto r = new Request();
-    r->req = *req;
-    std::memset(r->out, 0, sizeof(r->out));
+    return tucu_server_submit_many(server, req, 1, id);
+}
+
+int tucu_server_submit_many(tucu_server server, const tucu_request* reqs, int n, uint64_t* ids)
+{
+    if (!server || !reqs || n < 0) return fail(TUCU_ERR_INVALID, "null argument");
+    int rc;
+    for (int i = 0; i < n; ++i)
+        if ((rc = check_request(*server, reqs[i]))) return rc;
+    std::vector<Request*> rs(size_t(n), nullptr);
+    for (int i = 0; i < n; ++i) {
+        rs[size_t(i)] = new Request();
+        rs[size_t(i)]->req = reqs[i];
+        std::memset(rs[size_t(i)]->out, 0, sizeof(rs[size_t(i)]->out));
+    }
     {
         std::lock_guard<std::mutex> lock(server->mu);
-        if (!server->error.empty()) { delete r; return fail(TUCU_ERR_DEVICE, server->error); }
-        r->id = server->next_id++;
-        server->live[r->id] = r;
-        server->submitted.push_back(r);
+        if (!server->error.empty()) {
+            for (Request* r : rs) delete r;
+            return fail(TUCU_ERR_DEVICE, server->error);
+        }
+        for (Request* r : rs) {
+            r->id = server->next_id++;
+            server->live[r->id] = r;
+            server->submitted.push_back(r);
+        }
     }
     server->cv.notify_one();
-    if (id) *id = r->id;
+    if (ids)
+        for (int i = 0; i < n; ++i) ids[i] = rs[size_t(i)]->id;
     return TUCU_OK;
 }
 

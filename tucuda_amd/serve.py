@@ -24,9 +24,10 @@ from .filters import _DTYPES, image_desc
 class Server:
     def __init__(self, pool_bytes: int, in_dtype=torch.bfloat16, out_dtype=torch.bfloat16,
                  rule: int = _capi.TUCU_RULE_CEIL4SIGMA, tile_rows: int = 0, max_batch_jobs: int = 1024,
-                 max_inflight: int = 2, max_tiles: int = 0):
+                 max_inflight: int = 2, max_tiles: int = 0, max_batch_mpix: int = 0):
         cfg = TucuServerConfig(int(pool_bytes), _DTYPES[in_dtype], _DTYPES[out_dtype], int(rule),
-                               int(tile_rows), int(max_batch_jobs), int(max_inflight), int(max_tiles), 0)
+                               int(tile_rows), int(max_batch_jobs), int(max_inflight), int(max_tiles),
+                               int(max_batch_mpix))
         h = ctypes.c_void_p()
         check(_capi.load().tucu_server_create(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h
@@ -42,6 +43,21 @@ class Server:
         rid = ctypes.c_uint64(0)
         check(_capi.load().tucu_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
         return rid.value
+
+    def submit_many(self, requests, arrival: float | None = None) -> list:
+        """Submits ``(img, sigma, mask)`` or ``(img, sigma, mask, shift)``
+        tuples in one call (all checked first; none queued if one is
+        invalid); returns their ids in order."""
+        n = len(requests)
+        arr = (TucuRequest * max(n, 1))()
+        t = self.now() if arrival is None else float(arrival)
+        for i, r in enumerate(requests):
+            img, sigma, mask = r[0], r[1], r[2]
+            shift = r[3] if len(r) > 3 else True
+            arr[i] = TucuRequest(image_desc(img), float(sigma), int(mask), int(bool(shift)), 0, t)
+        ids = (ctypes.c_uint64 * max(n, 1))()
+        check(_capi.load().tucu_server_submit_many(self._h, arr, n, ids))
+        return list(ids[:n])
 
     def poll(self, max_items: int = 4096):
         n = ctypes.c_int(0)

@@ -11,10 +11,15 @@ the six outputs.  Requests arrive as a Poisson process at each rate for
 arrival time, so it includes any lag of the submitting loop.  Outputs are
 released as soon as they are polled.
 
-A capacity run then submits ``--capacity-requests`` requests at once (the
-pool's admission throttles them) and reports the served throughput, latency
-and batch sizes under that saturation next to
-the plain per-image forms on the same requests, float32 and bfloat16 images:
+Images and outputs have rows padded to 256 bytes (the ``cu::Array2``
+layout; the server's pool pads output rows to 128 bytes).
+
+A capacity run then submits ``--capacity-requests`` requests in one call to
+a server whose pool (``--capacity-pool-gib``) holds the whole burst's
+outputs, so the scheduler, not the submitting or draining Python loop, sets
+the pace; it reports the served throughput, latency and batch sizes under
+that saturation next to the plain per-image forms on the same requests,
+float32 and bfloat16 images:
 one fused launch per request, and one launch per requested output (the
 method-per-output calls of FIRGaussianConvolver2); all launches captured in
 one HIP graph, the best case for per-image launches (no host cost between
@@ -54,8 +59,10 @@ def make_images(n: int, seed: int, frac_4k: float = 0.2):
 
 
 def device_image(seed: int, h: int, w: int, dev, dtype, nedges: int = 8, amplitude: float = 64.0):
-    """Uniform [0,255) image plus ``nedges`` step edges, generated on the device."""
+    """Uniform [0,255) image plus ``nedges`` step edges, generated on the
+    device, in the layout of ``cu::Array2`` (rows padded to 256 bytes)."""
     import torch
+    import tucuda_amd
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     img = torch.rand((h, w), generator=g, device=dev, dtype=torch.float32) * 255.0
@@ -65,7 +72,16 @@ def device_image(seed: int, h: int, w: int, dev, dtype, nedges: int = 8, amplitu
     for t, cx, cy in par:
         th = 2.0 * math.pi * t
         img += amplitude * (((xx - cx * w) * math.cos(th) + (yy - cy * h) * math.sin(th)) > 0).float()
-    return img.to(dtype)
+    out = tucuda_amd.array2(h, w, dtype, dev)
+    out.copy_(img)
+    return out
+
+
+def _array2_like(x, dtype):
+    import tucuda_amd
+    y = tucuda_amd.array2(x.shape[0], x.shape[1], dtype, x.device)
+    y.copy_(x)
+    return y
 
 
 def make_requests(n: int, nimages: int, seed: int):
@@ -188,8 +204,10 @@ def run_stream(srv, images, rate: float, duration: float, seed: int, check_every
 
 
 def run_capacity(srv, images, n: int, seed: int):
-    """n requests submitted at once (each stamped with its own submission
-    time); served throughput in Gpix/s of input, and the latency and batch
+    """n requests submitted at once in one call (tucu_server_submit_many,
+    all stamped with the submission time): the scheduler sees a full queue,
+    so the served rate is the GPU's and the server's, not the submitting
+    thread's; served throughput in Gpix/s of input, and the latency and batch
     sizes under that saturation."""
     reqs = make_requests(n, len(images), seed)
     pix = sum(images[ii].shape[0] * images[ii].shape[1] for ii, _, _ in reqs)
@@ -206,9 +224,7 @@ def run_capacity(srv, images, n: int, seed: int):
             srv.release(c["id"])
 
     t0 = srv.now()
-    for ii, sigma, mask in reqs:
-        pending.add(srv.submit(images[ii], sigma, mask))
-        drain()
+    pending.update(srv.submit_many([(images[ii], sigma, mask) for ii, sigma, mask in reqs], arrival=t0))
     deadline = time.monotonic() + 120.0
     while pending and time.monotonic() < deadline:
         drain()
@@ -229,12 +245,13 @@ def plain_per_image(images, reqs, dtype, reps: int = 3, per_method: bool = False
     """ms for one fused launch per request (outputs of the request's mask),
     or with ``per_method`` one launch per requested output (the way the
     method-per-output API is called), every launch captured in one HIP
-    graph; float32 or bfloat16 images."""
+    graph; float32 or bfloat16 images, inputs and outputs with rows padded
+    to 256 bytes like the served form's."""
     import torch
     import tucuda_amd
     dev = images[0].device
-    xs = [x.to(dtype) for x in images]
-    maxpx = max(x.numel() for x in xs)
+    xs = [_array2_like(x, dtype) for x in images]
+    maxpx = max(x.stride(0) * x.shape[0] for x in xs)
     flat = [torch.empty(maxpx, dtype=dtype, device=dev) for _ in range(6)]
     convs = {}
     calls = []
@@ -243,7 +260,9 @@ def plain_per_image(images, reqs, dtype, reps: int = 3, per_method: bool = False
         h, w = x.shape
         if sigma not in convs:
             convs[sigma] = tucuda_amd.FIRGaussianConvolver2(sigma, rule=tucuda_amd.TUCU_RULE_CEIL4SIGMA)
-        outs = {m: flat[k][: h * w].view(h, w) for k, m in enumerate(tucuda_amd.METHOD_ORDER) if mask >> k & 1}
+        ld = x.stride(0)
+        outs = {m: flat[k][: h * ld].view(h, ld)[:, :w] for k, m in enumerate(tucuda_amd.METHOD_ORDER)
+                if mask >> k & 1}
         if per_method:
             calls.extend((convs[sigma], x, {m: t}) for m, t in outs.items())
         else:
@@ -291,6 +310,9 @@ def main(argv=None):
     ap.add_argument("--tile-rows", type=int, default=0, help="batched tile rows (0: the library default)")
     ap.add_argument("--max-inflight", type=int, default=2)
     ap.add_argument("--capacity-requests", type=int, default=2000)
+    ap.add_argument("--capacity-pool-gib", type=float, default=48.0,
+                    help="output pool of the capacity run's server (room for the whole burst's outputs)")
+    ap.add_argument("--max-batch-mpix", type=int, default=0, help="input megapixels per launch (0: the library default)")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
     args = ap.parse_args(argv)
 
@@ -304,14 +326,18 @@ def main(argv=None):
                           f"sigma U[0.8,4], radius ceil(4 sigma), random output subset",
               "pool_gib": args.pool_gib, "tile_rows": args.tile_rows, "max_inflight": args.max_inflight,
               "streams": []}
-    with Server(int(args.pool_gib * 2**30), tile_rows=args.tile_rows, max_inflight=args.max_inflight) as srv:
+    opts = dict(tile_rows=args.tile_rows, max_inflight=args.max_inflight, max_batch_mpix=args.max_batch_mpix)
+    with Server(int(args.pool_gib * 2**30), **opts) as srv:
         run_stream(srv, images, 500.0, 0.5, args.seed + 99, check_every=0)        # warm-up
         for k, rate in enumerate(float(r) for r in args.rates.split(",") if r):
             res = run_stream(srv, images, rate, args.duration, args.seed + 10 * k)
             result["streams"].append(res)
             print(json.dumps({"progress": res}), file=sys.stderr, flush=True)
-        if args.capacity_requests > 0:
+    if args.capacity_requests > 0:
+        with Server(int(args.capacity_pool_gib * 2**30), **opts) as srv:
+            run_capacity(srv, images, 200, args.seed + 98)                      # warm-up
             reqs, pix, cap = run_capacity(srv, images, args.capacity_requests, args.seed + 77)
+            cap["pool_gib"] = args.capacity_pool_gib
             result["capacity"] = cap
     if args.capacity_requests > 0 and not args.no_plain:
         cap = result["capacity"]
