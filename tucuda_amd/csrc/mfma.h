@@ -49,7 +49,9 @@ constexpr int MWAVES   = 4;               // strips (waves) per workgroup, side 
 constexpr int MTILE_H  = 16;              // centre rows per workgroup tile
 constexpr int MTILE_W  = MSTRIP_W * MWAVES;   // centre columns per workgroup tile
 constexpr int MOPITCH  = MSTRIP_W + 4;    // row pitch (floats) of a wave's output staging area
-constexpr int MT4      = 64;              // tap-run table entries per lobe
+constexpr int MT4      = 112;             // tap-run table entries per lobe
+constexpr int MBASE    = 64;              // run t starts with tap t - MBASE
+constexpr int MHP      = MT4 - 16;        // entries of the horizontal pair table
 constexpr int MINVALID = int(0x80000000u);   // buffer offset past any image: loads return 0
 
 template <int RC> struct MGeo {
@@ -60,7 +62,7 @@ template <int RC> struct MGeo {
 };
 
 // Per-lane operand runs, in LDS: run[(part * 3 + l) * MT4 + t] holds taps
-// t - 32 .. t - 29 of lobe l, times mp[l], as four f16 (part 0 the high
+// t - MBASE .. t - MBASE + 3 of lobe l, times mp[l], as four f16 (part 0 the high
 // half, part 1 the low half).  Tap o is the weight of the input at offset o
 // (row or column) from the centre, zero beyond the lobe: symmetric lobes (0,
 // 2) weigh both sides c[|o|]; the antisymmetric lobe 1 weighs the upper /
@@ -69,7 +71,8 @@ template <int RC> struct MGeo {
 // with 8-byte LDS reads and needs no conversion.
 struct MfmaTabs {
     v4f16 run[2 * 3 * MT4];
-    float tap[3][MT4 + 4];     // staging: tap o of lobe l times mp[l] at [l][o + 32]
+    v8f16 hpair[2 * 3 * MHP];  // horizontal operands: runs t and t + 16 side by side
+    float tap[3][MT4 + 4];     // staging: tap o of lobe l times mp[l] at [l][o + MBASE]
 };
 
 // MODE_GAUSS: the three lobes of the Gaussian (centre-indexed cv[l]).
@@ -100,7 +103,7 @@ __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
     constexpr int NT4 = MT4 + 4;
     for (int e = threadIdx.x; e < NL * NT4; e += blockDim.x) {
         const int l = e / NT4, t = e - l * NT4;
-        T.tap[l][t] = mfma_tap<MODE>(a, l, t - 32) * a.mp[l];
+        T.tap[l][t] = mfma_tap<MODE>(a, l, t - MBASE) * a.mp[l];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < NL * MT4; e += blockDim.x) {
@@ -115,6 +118,14 @@ __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
         }
         T.run[(0 * 3 + l) * MT4 + t] = hi;
         T.run[(1 * 3 + l) * MT4 + t] = lo;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NL * MHP; e += blockDim.x) {
+        const int l = e / MHP, t = e - l * MHP;
+#pragma unroll
+        for (int part = 0; part < 2; ++part)
+            T.hpair[(part * 3 + l) * MHP + t] = __builtin_shufflevector(
+                T.run[(part * 3 + l) * MT4 + t], T.run[(part * 3 + l) * MT4 + t + 16], 0, 1, 2, 3, 4, 5, 6, 7);
     }
 }
 
@@ -180,6 +191,13 @@ __device__ __forceinline__ v8f16 pair8(const V& f)
                                    4 * O + 4, 4 * O + 5, 4 * O + 6, 4 * O + 7);
 }
 
+// Field blocks 2 P and 2 P + 1 (pairs that do not overlap).
+template <int P, class V>
+__device__ __forceinline__ v8f16 pairP(const V& f)
+{
+    return pair8<2 * P>(f);
+}
+
 // f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled.
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f)
@@ -198,13 +216,16 @@ __device__ __forceinline__ void static_for(F&& f)
 // Operand layouts (lane = 16 g + m; m is also the row / output column n):
 //   vertical, block b, K32 slice: A = input rows y0 - RC + 8 g + j (j < 8)
 //     of column x0 - RC + 16 b + m; B = taps of offsets 8 g + j - RC - n.
-//   vertical, K48 second slice: A = entries 4 .. 11 of the lane's input,
-//     i.e. rows 8 g + 4 .. 8 g + 7 (weight 0) and 32 + 4 g .. +3; B = zeros
-//     and taps of offsets 32 + 4 g + j - RC - n.
-//   horizontal, output block o, K32 slice: B = field blocks o, o + 1 (lane
-//     holds columns 4 g .. 4 g + 3 of each, window columns 4 g + j and
-//     16 + 4 g + j); A = taps of offsets (window column) - RC - m.
-//   horizontal, K48 second slice: B = field blocks o + 1 (weight 0), o + 2.
+//   vertical, K48 second slice: A = rows 32 + 4 g .. +3 of the lane's
+//     input, high halves then low halves; B = taps of offsets
+//     32 + 4 g + j - RC - n twice (high taps: both halves of the input in
+//     one product), or once and then zeros (low taps x high input).
+//   horizontal, K32 (radius class 8), output block o: B = field blocks o,
+//     o + 1 (lane holds columns 4 g .. 4 g + 3 of each, window columns
+//     4 g + j and 16 + 4 g + j); A = taps of offsets (window column) - RC - m.
+//   horizontal, K48: two K32 slices over the fixed pairs P_(o/2) and
+//     P_(o/2 + 1) of field blocks (2 p, 2 p + 1); A = taps as above, shifted
+//     by d = 32 p - 16 o (zero for the block outside the window).
 // A strip's input, as loaded: field block b, entry j (mfma_load_block's
 // order); radius class 8 uses blocks 0 .. 4 and entries 0 .. 7.
 typedef float MfmaRaw[6][12];
@@ -238,20 +259,23 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
 
 // One strip: centre rows y0 .. y0+15, centre columns x0 .. x0+63, input in
 // raw (mfma_load).  T holds the operand runs (mfma_build_tabs), stage is the
-// wave's output staging area.  after_vpass() runs once the input has been
+// wave's output staging area.  after_vpass() runs once raw has been
 // consumed (it may reuse raw, e.g. to prefetch the next strip).  Every lane
 // of the wave calls it.
 //
 // Operand layouts (lane = 16 g + m; m is also the row / output column n):
 //   vertical, block b, K32 slice: A = input rows y0 - RC + 8 g + j (j < 8)
 //     of column x0 - RC + 16 b + m; B = taps of offsets 8 g + j - RC - n.
-//   vertical, K48 second slice: A = entries 4 .. 11 of the lane's input,
-//     i.e. rows 8 g + 4 .. 8 g + 7 (weight 0) and 32 + 4 g .. +3; B = zeros
-//     and taps of offsets 32 + 4 g + j - RC - n.
-//   horizontal, output block o, K32 slice: B = field blocks o, o + 1 (lane
-//     holds columns 4 g .. 4 g + 3 of each, window columns 4 g + j and
-//     16 + 4 g + j); A = taps of offsets (window column) - RC - m.
-//   horizontal, K48 second slice: B = field blocks o + 1 (weight 0), o + 2.
+//   vertical, K48 second slice: A = rows 32 + 4 g .. +3 of the lane's
+//     input, high halves then low halves; B = taps of offsets
+//     32 + 4 g + j - RC - n twice (high taps: both halves of the input in
+//     one product), or once and then zeros (low taps x high input).
+//   horizontal, K32 (radius class 8), output block o: B = field blocks o,
+//     o + 1 (lane holds columns 4 g .. 4 g + 3 of each, window columns
+//     4 g + j and 16 + 4 g + j); A = taps of offsets (window column) - RC - m.
+//   horizontal, K48: two K32 slices over the fixed pairs P_(o/2) and
+//     P_(o/2 + 1) of field blocks (2 p, 2 p + 1); A = taps as above, shifted
+//     by d = 32 p - 16 o (zero for the block outside the window).
 template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
 __device__ __forceinline__ void
 mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
@@ -262,7 +286,6 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     constexpr bool XLO = sizeof(IN) == 4;       // f16 / bf16 / u8 inputs are exact in one f16
     constexpr int NF = 4 * NVB;                 // field entries per lane
     typedef _Float16 vfield __attribute__((ext_vector_type(NF)));
-    typedef _Float16 vin __attribute__((ext_vector_type(12)));
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, m = lane & 15;
     const unsigned mask = a.mask;
@@ -280,35 +303,39 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     ex = ex < -100 ? -100 : ex > 100 ? 100 : ex;
     const float s = ldexpf(a.mq, ex), inv_s = ldexpf(1.f, -ex);
 
-    vin xh[NVB], xl[NVB];
+    // xh / xl: entries 0 .. 7 (rows 8 g + j), high and low halves; xt (K = 48):
+    // entries 8 .. 11 (rows 32 + 4 g + j), high then low halves, so that one
+    // product with the high taps twice covers both halves of those rows
+    v8f16 xh[NVB], xl[NVB], xt[NVB];
 #pragma unroll
     for (int b = 0; b < NVB; ++b) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            v4f16 h, l;
-            split4(raw[b][4 * q], raw[b][4 * q + 1], raw[b][4 * q + 2], raw[b][4 * q + 3], s, h, l);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) { xh[b][4 * q + i] = h[i]; xl[b][4 * q + i] = l[i]; }
+        v4f16 h0, l0, h1, l1;
+        split4(raw[b][0], raw[b][1], raw[b][2], raw[b][3], s, h0, l0);
+        split4(raw[b][4], raw[b][5], raw[b][6], raw[b][7], s, h1, l1);
+        xh[b] = cat8(h0, h1);
+        xl[b] = cat8(l0, l1);
+        if constexpr (G::K48) {
+            v4f16 h2, l2;
+            split4(raw[b][8], raw[b][9], raw[b][10], raw[b][11], s, h2, l2);
+            xt[b] = cat8(h2, l2);
         }
     }
+    after_vpass();
 
     // --------------------------------------------------- vertical pass
     // fh[l] / fl[l]: entries 4 b .. 4 b + 3 = field l at row y0 + n, columns
     // x0 - RC + 16 b + 4 g .. +3 (the accumulator layout), split in f16.
     // One lobe at a time, the chains of the field blocks interleaved.
-    vfield fh[3], fl[3];
-    const unsigned fneed[3] = {ONE ? 1u : mask & 0x0bu, ONE ? 0u : mask & 0x14u, ONE ? 0u : mask & 0x20u};
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-        if (!fneed[l]) continue;
-        const int t1 = 32 + 8 * g - RC - m;
+    auto vpass = [&](int l, vfield& fh, vfield& fl) {
+        const int t1 = MBASE + 8 * g - RC - m;
         const v8f16 ch1 = cat8(T.run[(0 * 3 + l) * MT4 + t1], T.run[(0 * 3 + l) * MT4 + t1 + 4]);
         const v8f16 cl1 = cat8(T.run[(1 * 3 + l) * MT4 + t1], T.run[(1 * 3 + l) * MT4 + t1 + 4]);
-        v8f16 ch2, cl2;
-        if constexpr (G::K48) {
-            const int t2 = 64 + 4 * g - RC - m;
-            ch2 = cat8(z4, T.run[(0 * 3 + l) * MT4 + t2]);
-            cl2 = cat8(z4, T.run[(1 * 3 + l) * MT4 + t2]);
+        v8f16 cht, clt;
+        if constexpr (G::K48) {                 // rows 32 + 4 g + j: high taps for both halves
+            const int t2 = MBASE + 32 + 4 * g - RC - m;
+            const v4f16 th = T.run[(0 * 3 + l) * MT4 + t2];
+            cht = cat8(th, th);
+            clt = cat8(T.run[(1 * 3 + l) * MT4 + t2], z4);
         }
         v4f32 acc[NVB];
 #pragma unroll
@@ -316,38 +343,26 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
         // smaller products first
         if constexpr (XLO) {
 #pragma unroll
-            for (int b = 0; b < NVB; ++b)
-                acc[b] = TUCU_MFMA(__builtin_shufflevector(xl[b], xl[b], 0, 1, 2, 3, 4, 5, 6, 7), ch1, acc[b]);
+            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xl[b], ch1, acc[b]);
         }
 #pragma unroll
-        for (int b = 0; b < NVB; ++b)
-            acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 0, 1, 2, 3, 4, 5, 6, 7), cl1, acc[b]);
+        for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xh[b], cl1, acc[b]);
         if constexpr (G::K48) {
-            if constexpr (XLO) {
 #pragma unroll
-                for (int b = 0; b < NVB; ++b)
-                    acc[b] = TUCU_MFMA(__builtin_shufflevector(xl[b], xl[b], 4, 5, 6, 7, 8, 9, 10, 11), ch2, acc[b]);
-            }
+            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], clt, acc[b]);   // hi x lo taps
 #pragma unroll
-            for (int b = 0; b < NVB; ++b)
-                acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 4, 5, 6, 7, 8, 9, 10, 11), cl2, acc[b]);
-#pragma unroll
-            for (int b = 0; b < NVB; ++b)
-                acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 4, 5, 6, 7, 8, 9, 10, 11), ch2, acc[b]);
+            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], cht, acc[b]);   // (hi + lo) x hi taps
         }
 #pragma unroll
-        for (int b = 0; b < NVB; ++b)
-            acc[b] = TUCU_MFMA(__builtin_shufflevector(xh[b], xh[b], 0, 1, 2, 3, 4, 5, 6, 7), ch1, acc[b]);
+        for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xh[b], ch1, acc[b]);
 #pragma unroll
         for (int b = 0; b < NVB; ++b) {
             v4f16 h, lo;
             split4(acc[b][0], acc[b][1], acc[b][2], acc[b][3], 1.f, h, lo);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) { fh[l][4 * b + i] = h[i]; fl[l][4 * b + i] = lo[i]; }
+            for (int i = 0; i < 4; ++i) { fh[4 * b + i] = h[i]; fl[4 * b + i] = lo[i]; }
         }
-    }
-
-    after_vpass();
+    };
 
     // ------------------------------------------------- horizontal pass
     const bool fast = y0 >= a.fy0 && y0 + 16 <= a.fy1 && x0 >= a.fx0 && x0 + MSTRIP_W <= a.fx1;
@@ -355,30 +370,44 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     constexpr int NOUT  = ONE ? 1 : 6;
     constexpr int HL[6] = {ONE ? 1 : 0, 1, 0, 2, 1, 0};   // horizontal table of output k
     constexpr int VL[6] = {0, 0, 1, 0, 1, 2};             // vertical field of output k
-#pragma unroll
-    for (int k = 0; k < NOUT; ++k) {
-        if (!(mask >> k & 1u)) continue;
+    auto hpass = [&](int k, const vfield& fh, const vfield& fl) {
         const int h = HL[k], v = VL[k];
-        const int t1 = 32 + 4 * g - RC - m;
-        const v8f16 ch1 = cat8(T.run[(0 * 3 + h) * MT4 + t1], T.run[(0 * 3 + h) * MT4 + t1 + 16]);
-        const v8f16 cl1 = cat8(T.run[(1 * 3 + h) * MT4 + t1], T.run[(1 * 3 + h) * MT4 + t1 + 16]);
-        v8f16 ch2, cl2;
-        if constexpr (G::K48) {
-            const int t2 = 64 + 4 * g - RC - m;
-            ch2 = cat8(z4, T.run[(0 * 3 + h) * MT4 + t2]);
-            cl2 = cat8(z4, T.run[(1 * 3 + h) * MT4 + t2]);
-        }
         v4f32 acc[MNB];
 #pragma unroll
         for (int o = 0; o < MNB; ++o) acc[o] = v4f32{0.f, 0.f, 0.f, 0.f};
-        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(cl1, pair8<o>(fh[v]), acc[o]); });
-        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fl[v]), acc[o]); });
         if constexpr (G::K48) {
-            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(cl2, pair8<o + 1>(fh[v]), acc[o]); });
-            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch2, pair8<o + 1>(fl[v]), acc[o]); });
-            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch2, pair8<o + 1>(fh[v]), acc[o]); });
+            // Fixed pairs of field blocks P_p = (2 p, 2 p + 1), so that no
+            // two B operands overlap: output block o takes P_(o/2) and
+            // P_(o/2 + 1), whose columns lie d = 32 p - 16 o from its own
+            // (d in {-16, 0, 16, 32}; taps beyond the lobe are zero).
+            v8f16 ah[4], al[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = MBASE + 16 * i - 16 + 4 * g - RC - m;
+                ah[i] = T.hpair[(0 * 3 + h) * MHP + t];
+                al[i] = T.hpair[(1 * 3 + h) * MHP + t];
+            }
+            // smaller products first, as in the vertical pass
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2, i = (o & 1) ? 0 : 1;
+                acc[o] = TUCU_MFMA(al[i], pairP<p>(fh), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2, i = (o & 1) ? 0 : 1;
+                acc[o] = TUCU_MFMA(ah[i], pairP<p>(fl), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2 + 1, i = (o & 1) ? 2 : 3;
+                acc[o] = TUCU_MFMA(al[i], pairP<p>(fh), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2 + 1, i = (o & 1) ? 2 : 3;
+                acc[o] = TUCU_MFMA(ah[i], pairP<p>(fl), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2 + 1, i = (o & 1) ? 2 : 3;
+                acc[o] = TUCU_MFMA(ah[i], pairP<p>(fh), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc, p = o / 2, i = (o & 1) ? 0 : 1;
+                acc[o] = TUCU_MFMA(ah[i], pairP<p>(fh), acc[o]); });
+        } else {
+            const int t1 = MBASE + 4 * g - RC - m;
+            const v8f16 ch1 = T.hpair[(0 * 3 + h) * MHP + t1];
+            const v8f16 cl1 = T.hpair[(1 * 3 + h) * MHP + t1];
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(cl1, pair8<o>(fh), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fl), acc[o]); });
+            static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fh), acc[o]); });
         }
-        static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fh[v]), acc[o]); });
         const float u = a.mu[k] * inv_s;
         if (fast) {
             // Through the wave's LDS staging area, so that each store writes
@@ -403,6 +432,19 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
                 store4<OUT>(a, k, false, rv, a.nrow - rv, rh, a.ncol - rh, y, x0 + 16 * o + 4 * g,
                             acc[o][0] * u, acc[o][1] * u, acc[o][2] * u, acc[o][3] * u);
         }
+    };
+
+    // One field at a time, each followed by the outputs it feeds, so that
+    // the first stores leave after a third of the vertical work.
+    const unsigned fneed[3] = {ONE ? 1u : mask & 0x0bu, ONE ? 0u : mask & 0x14u, ONE ? 0u : mask & 0x20u};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        if (!fneed[l]) continue;
+        vfield fh, fl;
+        vpass(l, fh, fl);
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k)
+            if (VL[k] == l && (mask >> k & 1u)) hpass(k, fh, fl);
     }
 }
 #undef TUCU_MFMA
