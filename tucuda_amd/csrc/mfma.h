@@ -72,60 +72,88 @@ template <int RC> struct MGeo {
 struct MfmaTabs {
     v4f16 run[2 * 3 * MT4];
     v8f16 hpair[2 * 3 * MHP];  // horizontal operands: runs t and t + 16 side by side
+    float lobe[2 * 33];        // staging: cv[l][k] at 17 l + k (GAUSS), tv / th at 33 l + o + 16 (TAPS)
     float tap[3][MT4 + 4];     // staging: tap o of lobe l times mp[l] at [l][o + MBASE]
 };
 
-// MODE_GAUSS: the three lobes of the Gaussian (centre-indexed cv[l]).
-// MODE_TAPS (one output of FIRFilter2 or of general taps): table 0 holds the
-// vertical taps, table 1 the horizontal ones, both as full taps with tap o
-// at index o + 16 of tv / th.
-template <int MODE, class A>
-__device__ __forceinline__ float mfma_tap(const A& a, int l, int o)
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled.
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// Tap o of lobe l from the staged lobe words (MfmaTabs::lobe).  MODE_GAUSS:
+// the three lobes of the Gaussian (centre-indexed cv[l]).  MODE_TAPS (one
+// output of FIRFilter2 or of general taps): table 0 holds the vertical taps,
+// table 1 the horizontal ones, both as full taps with tap o at index o + 16
+// of tv / th.
+template <int MODE>
+__device__ __forceinline__ float mfma_tap_lds(const MfmaTabs& T, int l, int o)
 {
     const int k = o < 0 ? -o : o;
     if (k > 16) return 0.f;
     if constexpr (MODE == MODE_TAPS) {
-        return l == 0 ? a.tv[o + 16] : l == 1 ? a.th[o + 16] : 0.f;
+        return l < 2 ? T.lobe[33 * l + o + 16] : 0.f;
     } else {
-        const float c = a.cv[l][k];
+        const float c = T.lobe[17 * l + k];
         return l != 1 ? c : o == 0 ? 0.f : o < 0 ? c : -c;
     }
 }
 
-// Every thread of the workgroup calls it (it holds a barrier); the caller
-// puts another barrier between it and the first use of T.  The taps are
-// staged in LDS first, one kernel-argument load per thread, so the build
-// costs one memory round trip rather than one per tap of a run.
+template <int MODE> constexpr int mfma_nlobes() { return MODE == MODE_TAPS ? 2 : 3; }
+
+// Builds the operand tables of one job in LDS.  Every thread of the
+// workgroup calls it (it holds a barrier); the caller puts another barrier
+// between it and the first use of T.  The lobe words are read with scalar
+// loads (their own counter, so vector loads issued before the call, such as
+// a strip's input, stay in flight) and written to LDS by one lane; every
+// entry is then built from LDS.
 template <int MODE = MODE_GAUSS, class A>
 __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
 {
-    constexpr int NL = MODE == MODE_TAPS ? 2 : 3;
+    constexpr int NL = mfma_nlobes<MODE>();
+    constexpr int NW = MODE == MODE_TAPS ? 66 : 51;
+    if (threadIdx.x == 0) {
+        static_for<0, NW>([&](auto ic) {
+            constexpr int i = ic;
+            if constexpr (MODE == MODE_TAPS) T.lobe[i] = i < 33 ? a.tv[i] : a.th[i - 33];
+            else                             T.lobe[i] = a.cv[i / 17][i % 17];
+        });
+    }
+    const float mp0 = a.mp[0], mp1 = a.mp[1], mp2 = a.mp[2];
+    __syncthreads();
     constexpr int NT4 = MT4 + 4;
     for (int e = threadIdx.x; e < NL * NT4; e += blockDim.x) {
         const int l = e / NT4, t = e - l * NT4;
-        T.tap[l][t] = mfma_tap<MODE>(a, l, t - MBASE) * a.mp[l];
+        T.tap[l][t] = mfma_tap_lds<MODE>(T, l, t - MBASE) * (l == 0 ? mp0 : l == 1 ? mp1 : mp2);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < NL * MT4; e += blockDim.x) {
-        const int l = e / MT4, t = e - l * MT4;
-        v4f16 hi, lo;
+    // runs (vertical operands: taps t - MBASE .. +3) and pairs of runs 16
+    // apart (horizontal operands)
+    for (int e = threadIdx.x; e < NL * (MT4 + MHP); e += blockDim.x) {
+        const bool pair = e >= NL * MT4;
+        const int ee = pair ? e - NL * MT4 : e;
+        const int n = pair ? MHP : MT4;
+        const int l = ee / n, t = ee - l * n;
+        v8f16 hi, lo;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float c = T.tap[l][t + i];
+        for (int i = 0; i < 8; ++i) {               // taps t .. t + 3, then (pairs) t + 16 .. t + 19
+            const float c = i < 4 || pair ? T.tap[l][t + (i < 4 ? i : 12 + i)] : 0.f;
             const _Float16 h = _Float16(c);
             hi[i] = h;
             lo[i] = _Float16(c - float(h));
         }
-        T.run[(0 * 3 + l) * MT4 + t] = hi;
-        T.run[(1 * 3 + l) * MT4 + t] = lo;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NL * MHP; e += blockDim.x) {
-        const int l = e / MHP, t = e - l * MHP;
-#pragma unroll
-        for (int part = 0; part < 2; ++part)
-            T.hpair[(part * 3 + l) * MHP + t] = __builtin_shufflevector(
-                T.run[(part * 3 + l) * MT4 + t], T.run[(part * 3 + l) * MT4 + t + 16], 0, 1, 2, 3, 4, 5, 6, 7);
+        if (pair) {
+            T.hpair[(0 * 3 + l) * MHP + t] = hi;
+            T.hpair[(1 * 3 + l) * MHP + t] = lo;
+        } else {
+            T.run[(0 * 3 + l) * MT4 + t] = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
+            T.run[(1 * 3 + l) * MT4 + t] = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
+        }
     }
 }
 
@@ -196,16 +224,6 @@ template <int P, class V>
 __device__ __forceinline__ v8f16 pairP(const V& f)
 {
     return pair8<2 * P>(f);
-}
-
-// f(std::integral_constant<int, I>) for I = 0 .. N-1, unrolled.
-template <int I, int N, class F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        static_for<I + 1, N>(f);
-    }
 }
 
 #define TUCU_MFMA(A_, B_, C_) __builtin_amdgcn_mfma_f32_16x16x32_f16((A_), (B_), (C_), 0, 0, 0)
