@@ -344,6 +344,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // fh[l] / fl[l]: entries 4 b .. 4 b + 3 = field l at row y0 + n, columns
     // x0 - RC + 16 b + 4 g .. +3 (the accumulator layout), split in f16.
     // One lobe at a time, the chains of the field blocks interleaved.
+    // 1.0 in a register whose value the compiler does not see: v * one - hi
+    // then stays one v_fma_mix (v - hi alone would be a convert, a subtract
+    // and another convert)
+    float one = 1.f;
+    asm("" : "+v"(one));
     auto vpass = [&](int l, vfield& fh, vfield& fl) {
         const int t1 = MBASE + 8 * g - RC - m;
         const v8f16 ch1 = cat8(T.run[(0 * 3 + l) * MT4 + t1], T.run[(0 * 3 + l) * MT4 + t1 + 4]);
@@ -376,7 +381,10 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
 #pragma unroll
         for (int b = 0; b < NVB; ++b) {
             v4f16 h, lo;
-            split4(acc[b][0], acc[b][1], acc[b][2], acc[b][3], 1.f, h, lo);
+            // hi = f16(v), lo = f16(v - hi): one mixed-precision fma each
+            h = __builtin_convertvector(acc[b], v4f16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) lo[i] = _Float16(__builtin_fmaf(acc[b][i], one, -float(h[i])));
 #pragma unroll
             for (int i = 0; i < 4; ++i) { fh[4 * b + i] = h[i]; fl[4 * b + i] = lo[i]; }
         }
