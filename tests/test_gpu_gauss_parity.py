@@ -108,3 +108,23 @@ def test_workload_1080p_six(dev, kernel_family):
     tucuda_amd.FIRGaussianConvolver2(3.0).convolve6(to_dev(img, dev), outs)
     for m in oracle.METHOD_ORDER:
         assert_f32_parity(to_host(outs[m]), refs[m], f"1080p {m}")
+
+
+@pytest.mark.parametrize("sigma", [1.2, 3.0])
+@pytest.mark.parametrize("ik", ["f32", "u8"])
+def test_fused_six_large_image(dev, sigma, ik):
+    """An image with more tiles than resident workgroups (1512 x 2616, about
+    1000 tiles), which the matrix-core path filters with its tile loop (each
+    workgroup builds the tables once and takes a share of the tiles); both
+    radius classes and both input widths, against the oracle."""
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(14, 1512, 2616)
+    if ik == "u8":
+        img = np.clip(img, 0, 255).astype(np.uint8)
+    refs = _oracle_all(sigma, img)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    outs = {m: torch.full(img.shape, sentinel(np.float32), device=dev) for m in oracle.METHOD_ORDER}
+    g.convolve6(to_dev(img, dev), outs)
+    for m in oracle.METHOD_ORDER:
+        assert_f32_parity(to_host(outs[m]), refs[m], f"large sigma={sigma} {ik} {m}")
