@@ -642,7 +642,7 @@ batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict
 // the waves of a SIMD drift apart and their loads, matrix work and stores
 // overlap.  Each wave keeps its own tap tables.
 template <class IN, class OUT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MFMA_WAVES_PER_SIMD<IN>)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))   // LDS allows no more
 batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict__ tiles, int ntiles,
                        int* __restrict__ counter)
 {
