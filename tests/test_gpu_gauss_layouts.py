@@ -84,3 +84,19 @@ def test_type_pairs_on_unaligned_views(dev, kernel_family, ik, ok):
         outside = np.ones(got_big.shape, bool)
         outside[2:, 1 + k:1 + k + ncol] = False
         assert (got_big[outside] == bigs[k][outside]).all(), (ik, ok, m)
+
+
+@pytest.mark.parametrize("dtype_name", ["uint8", "float32", "float16", "bfloat16"])
+def test_array2_layout(dev, dtype_name):
+    """tucuda_amd.array2: the cu::Array2 layout (rows padded to a multiple of
+    256 bytes, zero-filled), a view the filters take like any other."""
+    import torch
+    import tucuda_amd
+    dt = getattr(torch, dtype_name)
+    t = tucuda_amd.array2(37, 101, dt, dev)
+    es = t.element_size()
+    assert tuple(t.shape) == (37, 101) and t.stride(1) == 1
+    assert (t.stride(0) * es) % 256 == 0 and t.stride(0) * es >= 101 * es
+    assert t.stride(0) * es - 101 * es < 256
+    assert (t.data_ptr() % 256) == 0
+    assert bool((t.float() == 0).all())
