@@ -13,7 +13,7 @@
 */
 #pragma once
 
-#include "TU/cu/Array++.h"
+#include "TU/cu/algorithm.h"
 
 #include <iterator>
 #include <vector>
@@ -22,23 +22,6 @@ namespace TU
 {
 namespace cu
 {
-//! Thread-block shape of the reference's kernels (algorithm.h:58-67); kept
-//! as the template parameter of FIRFilter2 for source compatibility.  The
-//! gfx950 kernels choose their own tiling.
-template <size_t BLOCK_DIM_X = 32, size_t BLOCK_DIM_Y = 16>
-struct BlockTraits
-{
-    constexpr static size_t BlockDimX = BLOCK_DIM_X;
-    constexpr static size_t BlockDimY = BLOCK_DIM_Y;
-    constexpr static size_t BlockDim  = BlockDimX * BlockDimY;
-};
-
-inline size_t
-divUp(size_t dim, size_t blockDim)
-{
-    return (dim + blockDim - 1) / blockDim;
-}
-
 /************************************************************************
 *  class FIRFilter2<T, BLOCK_TRAITS>                                    *
 ************************************************************************/

@@ -161,6 +161,38 @@ TUCU_API int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1
                                   const tucu_image* in, const tucu_image* outs,
                                   unsigned mask, int shift, void* stream);
 
+/* ---- subsample: the pyramid step ----------------------------------------- */
+
+/* Stencil operators of cu::subsample (TU/cu/functional.h). */
+enum {
+    TUCU_OP_IDENTITY1X1  = 0,   /* identity1x1   functional.h:58-65 */
+    TUCU_OP_DIFFH1X3     = 1,   /* diffH1x3      :68-81 */
+    TUCU_OP_DIFFV3X1     = 2,   /* diffV3x1      :84-98 */
+    TUCU_OP_DIFFHH1X3    = 3,   /* diffHH1x3     :101-114 */
+    TUCU_OP_DIFFVV3X1    = 4,   /* diffVV3x1     :117-130 */
+    TUCU_OP_DIFFHV3X3    = 5,   /* diffHV3x3     :133-148 */
+    TUCU_OP_SOBELH3X3    = 6,   /* sobelH3x3     :151-167 */
+    TUCU_OP_SOBELV3X3    = 7,   /* sobelV3x3     :170-186 */
+    TUCU_OP_SOBELABS3X3  = 8,   /* sobelAbs3x3   :189-198 */
+    TUCU_OP_LAPLACIAN3X3 = 9,   /* laplacian3x3  :201-210 */
+    TUCU_OP_DET3X3       = 10,  /* det3x3        :213-226 */
+    TUCU_OP_GAUSSIAN5X5  = 11   /* gaussian5x5   :229-256 */
+};
+
+/* cu::subsample(in, ie, out, op) (TU/cu/algorithm.h:183-242): halves the
+ * image in both directions, out[y][x] = op evaluated at input pixel
+ * (2y, 2x), for y < nrow/2 and x < ncol/2.  The operator's border rule is
+ * that of its guarded form in functional.h: 0 wherever its stencil would
+ * leave the image.  Images with fewer than 2 rows or columns are a silent
+ * no-op (algorithm.h:227-233).  identity1x1 takes any tucu_dtype input; the
+ * other operators compute in FP32 and need a floating-point input (f32, or
+ * f16 / bf16 storage as an extension).  out may be any tucu_dtype (float ->
+ * u_char saturates, then truncates); if out->nrow / out->ncol are nonzero
+ * they must hold nrow/2 x ncol/2.  Exactly nrow/2 x ncol/2 pixels are
+ * written: for an odd extent the reference's grid can also write row nrow/2
+ * or column ncol/2, past the output it is sized for. */
+TUCU_API int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* stream);
+
 /* ---- batched mode (extension) ------------------------------------------
  * Many FIRGaussianConvolver2 jobs of uneven sizes and lobe radii in one
  * persistent launch.  No counterpart in the reference, which filters one

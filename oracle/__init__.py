@@ -36,6 +36,10 @@ METHODS = {
 }
 METHOD_ORDER = ("smooth", "diffH", "diffV", "diffHH", "diffHV", "diffVV")
 
+# cu::subsample's stencil operators (functional.h:58-256), in code order.
+SUBSAMPLE_OPS = ("identity1x1", "diffH1x3", "diffV3x1", "diffHH1x3", "diffVV3x1", "diffHV3x3",
+                 "sobelH3x3", "sobelV3x3", "sobelAbs3x3", "laplacian3x3", "det3x3", "gaussian5x5")
+
 
 class OracleError(RuntimeError):
     pass
@@ -73,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.orc_fir2_f64.argtypes = [f64p, ctypes.c_int, f64p, ctypes.c_int] + common
         L.orc_fir2_taps_f32.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_int] + common
         L.orc_fir2_taps_f64.argtypes = [f64p, ctypes.c_int, f64p, ctypes.c_int] + common
+        L.orc_subsample_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
         L.orc_half_to_float.argtypes = [ctypes.c_uint16]
         L.orc_half_to_float.restype = ctypes.c_float
         L.orc_float_to_half.argtypes = [ctypes.c_float]
@@ -190,6 +196,25 @@ def gauss(method: str, sigma: float, img: np.ndarray, out: np.ndarray, shift: bo
         lobes = gauss_lobes(sigma, precision)
     h, v = METHODS[method]
     return fir2(lobes[h], lobes[v], img, out, shift, precision, nthreads)
+
+
+def subsample(img: np.ndarray, out: np.ndarray, op) -> np.ndarray:
+    """cu::subsample(img, out, op) (algorithm.h:183-242): ``out[y][x]`` = the
+    operator at ``img[2y][2x]`` for y < nrow/2, x < ncol/2, 0 where its
+    stencil leaves the image (functional.h:58-256).  ``op`` is an index or a
+    name of SUBSAMPLE_OPS.  Writes ``out`` in place and returns it."""
+    code = SUBSAMPLE_OPS.index(op) if isinstance(op, str) else int(op)
+    assert img.ndim == 2 and out.ndim == 2
+    if img.strides[1] != img.itemsize or out.strides[1] != out.itemsize:
+        raise ValueError("rows must be contiguous")
+    nrow, ncol = img.shape
+    if out.shape[0] < nrow // 2 or out.shape[1] < ncol // 2:
+        raise ValueError("output must hold nrow/2 x ncol/2")
+    rc = lib().orc_subsample_f32(img.ctypes.data, _dt(img), img.strides[0] // img.itemsize, nrow, ncol,
+                                 out.ctypes.data, _dt(out), out.strides[0] // out.itemsize, code)
+    if rc != 0:
+        raise OracleError(f"oracle returned {rc}")
+    return out
 
 
 def full_kernel(lobe) -> np.ndarray:

@@ -23,6 +23,8 @@
 //   FIRGaussianConvolver.cu:52-98         detail::lobeSize (1 % threshold, size classes)
 //   TU/cu/algorithm.h:119-144             element conversion on tile load
 //   TU/cu/fp16.h:55-80                    round-to-nearest-even half conversion
+//   TU/cu/algorithm.h:183-242             subsample: out[y][x] = op at (2y, 2x), early return
+//   TU/cu/functional.h:58-256             the stencil operators and their border guards
 //
 // Status codes: 0 ok, 2 lobe longer than 17, 3 unsupported lobe length,
 // 1 invalid argument.
@@ -253,6 +255,69 @@ int fir2_taps(const A* tapsH, int nH, const A* tapsV, int nV,
     return 0;
 }
 
+// cu::subsample's operators (functional.h:58-256) at input pixel (y, x) of
+// an FP32 image: the guarded operator() of each, i.e. 0 unless the stencil
+// fits inside the image, then the source expression in source order.
+float subsample_op(int op, const std::vector<float>& im, int nrow, int ncol, int y, int x)
+{
+    static const int RY[12] = {0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 2};
+    static const int RX[12] = {0, 1, 0, 1, 0, 1, 1, 1, 1, 1, 1, 2};
+    const int ry = RY[op], rx = RX[op];
+    if (y < ry || y + ry >= nrow || x < rx || x + rx >= ncol) return 0.f;
+    auto in = [&](int yy, int xx) { return im[size_t(yy) * ncol + xx]; };
+    auto dHH = [&] { return in(y, x - 1) - 2.f * in(y, x) + in(y, x + 1); };        // :104-108
+    auto dVV = [&] { return in(y - 1, x) - 2.f * in(y, x) + in(y + 1, x); };        // :120-124
+    auto dHV = [&] {                                                               // :136-141
+        return 0.25f * (in(y - 1, x - 1) - in(y - 1, x + 1) - in(y + 1, x - 1) + in(y + 1, x + 1));
+    };
+    auto sH = [&] {                                                                // :154-159
+        return 0.125f * (in(y - 1, x + 1) - in(y - 1, x - 1) + in(y + 1, x + 1) - in(y + 1, x - 1))
+             + 0.250f * (in(y, x + 1) - in(y, x - 1));
+    };
+    auto sV = [&] {                                                                // :173-178
+        return 0.125f * (in(y + 1, x - 1) - in(y - 1, x - 1) + in(y + 1, x + 1) - in(y - 1, x + 1))
+             + 0.250f * (in(y + 1, x) - in(y - 1, x));
+    };
+    // gaussian5x5::smooth (:246-253), weights {0.375, 0.25, 0.0625}
+    auto g5 = [](const float* a, int i) {
+        return 0.375f * a[i] + 0.25f * (a[i - 1] + a[i + 1]) + 0.0625f * (a[i - 2] + a[i + 2]);
+    };
+    switch (op) {
+      case 0:  return in(y, x);                                                    // :61-64
+      case 1:  return 0.5f * (in(y, x + 1) - in(y, x - 1));                         // :71-74
+      case 2:  return 0.5f * (in(y + 1, x) - in(y - 1, x));                         // :87-90
+      case 3:  return dHH();
+      case 4:  return dVV();
+      case 5:  return dHV();
+      case 6:  return sH();
+      case 7:  return sV();
+      case 8:  return std::fabs(sH()) + std::fabs(sV());                           // :192-197
+      case 9:  return in(y, x - 1) + in(y, x + 1) + in(y - 1, x) + in(y + 1, x) - 4.f * in(y, x);   // :204-209
+      case 10: { const float dxy = dHV(); return dHH() * dVV() - dxy * dxy; }      // :216-225
+      default: {                                                                   // :233-244
+        float b[5];
+        for (int i = 0; i < 5; ++i) b[i] = g5(&im[size_t(y - 2 + i) * ncol], x);
+        return g5(b, 2);
+      }
+    }
+}
+
+// cu::subsample(in, ie, out, op) (algorithm.h:183-242): out[y][x] = op at
+// (2y, 2x) for y < nrow/2, x < ncol/2; fewer than 2 rows or columns: no-op.
+int subsample(const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+              void* out, int out_dt, int64_t out_stride, int op)
+{
+    if (nrow < 0 || ncol < 0 || op < 0 || op > 11) return 1;
+    if (nrow < 2 || ncol < 2) return 0;                                    // :227-233
+    std::vector<float> im(size_t(nrow) * ncol);
+    for (int y = 0; y < nrow; ++y)
+        for (int x = 0; x < ncol; ++x) im[size_t(y) * ncol + x] = load_px<float>(in, in_dt, int64_t(y) * in_stride + x);
+    for (int y = 0; y < nrow / 2; ++y)
+        for (int x = 0; x < ncol / 2; ++x)
+            store_px<float>(out, out_dt, int64_t(y) * out_stride + x, subsample_op(op, im, nrow, ncol, 2 * y, 2 * x));
+    return 0;
+}
+
 // detail::lobeSize (FIRGaussianConvolver.cu:52-98), generic in the element type.
 template <class A> size_t lobe_size(const A lobe[], bool even)
 {
@@ -361,6 +426,12 @@ int orc_fir2_taps_f64(const double* tapsH, int nH, const double* tapsV, int nV,
 {
     return fir2_taps<double>(tapsH, nH, tapsV, nV, in, in_dt, in_stride, nrow, ncol,
                              out, out_dt, out_stride, shift, nthreads);
+}
+
+int orc_subsample_f32(const void* in, int in_dt, int64_t in_stride, int nrow, int ncol,
+                      void* out, int out_dt, int64_t out_stride, int op)
+{
+    return subsample(in, in_dt, in_stride, nrow, ncol, out, out_dt, out_stride, op);
 }
 
 // Element conversions, exposed so the tests can pin them against numpy.

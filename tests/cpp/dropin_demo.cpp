@@ -14,6 +14,7 @@
 //                          (filterTest)
 // and prints one line of JSON with timings and the exception checks.
 #include "TU/cu/FIRGaussianConvolver.h"
+#include "TU/cu/algorithm.h"
 
 #include <chrono>
 #include <cmath>
@@ -97,6 +98,11 @@ main(int argc, char* argv[])
         cudaFilter.diffVV(in_d.cbegin(), in_d.cend(), outs[5].begin());
         for (int k = 0; k < 6; ++k) save(dir + "/" + NAMES[k] + "_f32.pfm", outs[k]);
 
+        // the pyramid step (Testprog/subsampleTest/main.cu:28-31 after a smooth)
+        cu::Array2<float> half(in_d.nrow() / 2, in_d.ncol() / 2);
+        cu::subsample(outs[0].cbegin(), outs[0].cend(), half.begin(), cu::identity1x1());
+        save(dir + "/half_smooth.pfm", half);
+
         // the six from one fused launch
         std::array<Out, 6> fused;
         std::array<Out::iterator, 6> its;
@@ -112,6 +118,11 @@ main(int argc, char* argv[])
         cu::Array2<float> outf_d(in.nrow(), in.ncol());
         fir.convolve(inf_d.cbegin(), inf_d.cend(), outf_d.begin());
         save(dir + "/fir_f32.pfm", outf_d);
+
+        // one more subsample operator, on the float upload
+        cu::Array2<float> g5(inf_d.nrow() / 2, inf_d.ncol() / 2);
+        cu::subsample(inf_d.cbegin(), inf_d.cend(), g5.begin(), cu::gaussian5x5());
+        save(dir + "/gaussian5x5_half.pfm", g5);
 
         // exceptions of the reference (FIRFilter.h:250-251, 344, 375)
         std::string too_large, unsupported;

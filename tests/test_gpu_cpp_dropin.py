@@ -74,3 +74,13 @@ def test_dropin_demo(dev, tmp_path):
     ref = np.zeros(img.shape, np.float32)
     oracle.fir2(coeff, coeff, img.astype(np.float32), ref)
     assert_f32_parity(read_pnm(tmp_path / "fir_f32.pfm"), ref, "drop-in FIRFilter2")
+
+    # the pyramid step: subsample(identity1x1) of the drop-in's own smooth
+    # output is its exact decimation; gaussian5x5 of the float upload equals
+    # the oracle bit for bit (no contraction on either side)
+    smooth = read_pnm(tmp_path / "smooth_f32.pfm")
+    half = read_pnm(tmp_path / "half_smooth.pfm")
+    np.testing.assert_array_equal(half, smooth[0:150:2, 0:260:2])
+    ref = np.zeros((75, 130), np.float32)
+    oracle.subsample(img.astype(np.float32), ref, "gaussian5x5")
+    np.testing.assert_array_equal(read_pnm(tmp_path / "gaussian5x5_half.pfm"), ref)

@@ -22,12 +22,16 @@ TUCU_U8, TUCU_F32, TUCU_F16, TUCU_BF16 = 0, 1, 2, 3
 TUCU_RULE_REFERENCE, TUCU_RULE_CEIL4SIGMA = 0, 1
 TUCU_ALL6 = 0x3F
 
+# cu::subsample operators (functional.h:58-256), TUCU_OP_* in include/tucu.h.
+SUBSAMPLE_OPS = ("identity1x1", "diffH1x3", "diffV3x1", "diffHH1x3", "diffVV3x1", "diffHV3x3",
+                 "sobelH3x3", "sobelV3x3", "sobelAbs3x3", "laplacian3x3", "det3x3", "gaussian5x5")
+
 # Every symbol include/tucu.h declares.
 EXPORTS = (
     "tucu_last_error", "tucu_version", "tucu_malloc", "tucu_free", "tucu_row_stride",
     "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
     "tucu_fir2_convolve_taps",
-    "tucu_lobe_size", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
+    "tucu_lobe_size", "tucu_subsample", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
     "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
     "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
@@ -167,6 +171,7 @@ def load() -> ctypes.CDLL:
     L.tucu_fir2_convolve_taps.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, imgp, imgp,
                                           ctypes.c_int, ctypes.c_void_p]
     L.tucu_lobe_size.argtypes = [fp, ctypes.c_int]
+    L.tucu_subsample.argtypes = [imgp, imgp, ctypes.c_int, ctypes.c_void_p]
     L.tucu_gauss2_lobes.argtypes = [ctypes.c_float, ctypes.c_int, fp, ip, fp, ip, fp, ip]
     L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,
                                        imgp, imgp, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]

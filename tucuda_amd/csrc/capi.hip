@@ -9,6 +9,7 @@
 #include "tucu.h"
 #include "sepfir.h"
 #include "mfma.h"
+#include "subsample.h"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -25,6 +26,7 @@ KernelEntry g_taps_table[4][4][4][4];
 KernelEntry g_gauss_table[5][NGTH][4][4];
 KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 KernelEntry g_fir_mfma_table[2][4][4];
+SubLaunchFn g_sub_table[NSUBOPS][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
@@ -536,6 +538,40 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
     const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
     if (!e->fn) e = &g_gauss_table[gauss_class(rmax)][GAUSS_TH_DEFAULT][in->dtype][odt];
     return launch(*e, a, static_cast<hipStream_t>(stream));
+}
+
+int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* stream)
+{
+    int rc;
+    if (op < 0 || op >= NSUBOPS) return fail(TUCU_ERR_INVALID, "unknown subsample operator");
+    if ((rc = check_image(in, "input", false))) return rc;
+    if ((rc = check_image(out, "output", false))) return rc;
+    const int nrow = in->nrow, ncol = in->ncol;
+    if (nrow < 2 || ncol < 2) return TUCU_OK;                       // algorithm.h:227-233
+    if (op != SUB_IDENTITY && in->dtype == TUCU_U8)
+        return fail(TUCU_ERR_INVALID, "subsample: this operator needs a floating-point input");
+    if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
+    if ((out->nrow > 0 && out->nrow < nrow / 2) || (out->ncol > 0 && out->ncol < ncol / 2))
+        return fail(TUCU_ERR_INVALID, "output: smaller than nrow/2 x ncol/2");
+    if (out->stride > INT32_MAX) return fail(TUCU_ERR_INVALID, "output row stride too large");
+    const int64_t es = int64_t(dtype_size(in->dtype));
+    const int64_t stride_b = in->stride * es;
+    if (int64_t(nrow) * stride_b > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+
+    SubArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data;
+    a.in_stride_b = int32_t(stride_b);
+    a.in_bytes = uint32_t(int64_t(nrow - 1) * stride_b + ncol * es);
+    a.nrow = nrow; a.ncol = ncol;
+    a.onrow = nrow / 2; a.oncol = ncol / 2;
+    a.vec = reinterpret_cast<uintptr_t>(in->data) % uintptr_t(4 * es) == 0 && stride_b % (4 * es) == 0;
+    a.ostride = int32_t(out->stride);
+    a.out = reinterpret_cast<uintptr_t>(out->data);
+    const SubLaunchFn fn = g_sub_table[op][in->dtype][out->dtype];
+    if (!fn) return fail(TUCU_ERR_INVALID, "no subsample kernel for this operator / type combination");
+    const hipError_t e = fn(a, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
 }
 
 }   // extern "C"

@@ -1,7 +1,9 @@
-"""Python mirror of TU::cu::FIRFilter2 and TU::cu::FIRGaussianConvolver2.
+"""Python mirror of TU::cu::FIRFilter2, TU::cu::FIRGaussianConvolver2 and
+TU::cu::subsample.
 
 Same method names, argument meanings and error behaviour as the C++
-classes (TU/cu/FIRFilter.h:18-52, TU/cu/FIRGaussianConvolver.h:56-90);
+classes and function (TU/cu/FIRFilter.h:18-52, TU/cu/FIRGaussianConvolver.h:56-90,
+TU/cu/algorithm.h:183-242);
 images are 2-D device tensors (rows contiguous, any row stride) standing in
 for ``cu::Array2<T>`` row ranges.  All work is enqueued on torch's current
 HIP stream through the C ABI; nothing here computes on the host.
@@ -143,6 +145,22 @@ class SeparableFilter2:
         check(_capi.load().tucu_fir2_convolve_taps(
             _fptr(self._th), self._th.size, _fptr(self._tv), self._tv.size,
             ctypes.byref(src), ctypes.byref(dst), int(bool(shift)), _stream_handle(inp)))
+
+
+SUBSAMPLE_OPS = _capi.SUBSAMPLE_OPS
+
+
+def subsample(inp: torch.Tensor, out: torch.Tensor, op="identity1x1") -> None:
+    """cu::subsample(in, ie, out, op) (algorithm.h:183-242): ``out[y][x]`` =
+    the operator (a name of SUBSAMPLE_OPS or its index, functional.h:58-256)
+    at ``inp[2y][2x]`` for y < nrow/2, x < ncol/2, 0 where its stencil would
+    leave the image.  Images with fewer than 2 rows or columns are a no-op.
+    Operators other than identity1x1 need a floating-point input."""
+    code = SUBSAMPLE_OPS.index(op) if isinstance(op, str) else int(op)
+    if inp.device != out.device:
+        raise ValueError("input and output must be on the same device")
+    src, dst = image_desc(inp), image_desc(out)
+    check(_capi.load().tucu_subsample(ctypes.byref(src), ctypes.byref(dst), code, _stream_handle(inp)))
 
 
 def gauss_lobes(sigma: float, rule: int = _capi.TUCU_RULE_REFERENCE):
