@@ -91,6 +91,19 @@ class FIRGaussianConvolver2 : public FIRFilter2<T>
                                    shift ? 1 : 0, super::stream()));
     }
 
+    //! Extension: the pyramid step in one launch.  out (nrow/2 x ncol/2)
+    //! receives what smooth() into a zero-filled image followed by
+    //! subsample(..., identity1x1()) gives, without the full-size
+    //! intermediate.
+    template <class IN, class OUT>
+    void smoothSubsample(IN in, IN ie, OUT out, bool shift = true) const
+    {
+        const tucu_image src = make_image(in, ie);
+        const tucu_image dst = make_out_image(out);
+        check(tucu_gauss2_pyramid(_lobe0.data(), int(_lobe0.size()), &src, &dst, shift ? 1 : 0,
+                                  super::stream()));
+    }
+
   private:
     template <class IN, class OUT>
     void apply(Method m, IN in, IN ie, OUT out, bool shift)

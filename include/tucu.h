@@ -193,6 +193,18 @@ enum {
  * or column ncol/2, past the output it is sized for. */
 TUCU_API int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* stream);
 
+/* Extension (no counterpart in the reference): the pyramid step fused.
+ * Equals FIRGaussianConvolver2::smooth(in -> tmp, shift) into a zero-filled
+ * nrow x ncol image tmp followed by subsample(tmp -> out, identity1x1)
+ * (FIRGaussianConvolver.h:178-182, algorithm.h:183-242), without writing or
+ * reading tmp: out[y][x] = the smoothed pixel at (2y, 2x), or 0 where that
+ * pixel lies outside smooth's valid region.  lobe0 is a symmetric lobe of
+ * n0 = r + 1 entries (centre last), r in 1..16, as for tucu_gauss2_convolve.
+ * Every one of the nrow/2 x ncol/2 output pixels is written; images with
+ * fewer than 2 rows or columns are a no-op.  Any tucu_dtype in and out. */
+TUCU_API int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in,
+                                 const tucu_image* out, int shift, void* stream);
+
 /* ---- batched mode (extension) ------------------------------------------
  * Many FIRGaussianConvolver2 jobs of uneven sizes and lobe radii in one
  * persistent launch.  No counterpart in the reference, which filters one

@@ -142,3 +142,101 @@ def test_errors(dev):
         tucuda_amd.subsample(xf, o, 12)
     tucuda_amd.subsample(x8, torch.zeros((8, 8), dtype=torch.uint8, device=dev), "identity1x1")
     torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+# The fused pyramid step (tucu_gauss2_pyramid, FIRGaussianConvolver2::
+# smoothSubsample): equal to smooth into a zero-filled image of the output's
+# element type followed by subsample(identity1x1), to the FP32 bar.
+
+def _pyr_expected(img, sigma, rule, ok, shift):
+    import tucuda_amd
+    if rule == 0:
+        lobe = oracle.gauss_lobes(sigma)[0]
+        smooth = lambda out: oracle.fir2(lobe, lobe, img, out, shift)          # noqa: E731
+    else:   # ceil(4 sigma) radii: the same symmetric kernel (lobe 0 = r + 1 entries, centre last) as general taps
+        lobe = tucuda_amd.FIRGaussianConvolver2.lobes(sigma, rule)[0]
+        taps = np.concatenate([lobe[:-1], lobe[-1:], lobe[:-1][::-1]]).astype(np.float32)
+        smooth = lambda out: oracle.fir2_taps(taps, taps, img, out, shift)     # noqa: E731
+    nrow, ncol = img.shape
+    full_f = np.zeros((nrow, ncol), np.float32)
+    smooth(full_f)
+    half_f = np.zeros((nrow // 2, ncol // 2), np.float32)
+    oracle.subsample(full_f, half_f, "identity1x1")
+    full = np.zeros((nrow, ncol), TYPES[ok])
+    smooth(full)
+    half = np.full((nrow // 2, ncol // 2), sentinel(TYPES[ok]), TYPES[ok])
+    oracle.subsample(full, half, "identity1x1")
+    return half, half_f
+
+
+def _pyr_check(got, ref, ref_f, ok, what):
+    from tests._util import assert_f32_parity, assert_u8_parity
+    if ok == "f32":
+        assert_f32_parity(got, ref, what)
+    elif ok == "u8":
+        assert_u8_parity(got, ref, ref_f, what)
+    else:
+        g = (got.astype(np.uint32) << 16).view(np.float32) if ok == "bf16" else got.astype(np.float32)
+        r = (ref.astype(np.uint32) << 16).view(np.float32) if ok == "bf16" else ref.astype(np.float32)
+        ulp = np.abs(r.astype(np.float64)) * (2.0 ** -10 if ok == "f16" else 2.0 ** -7)
+        tol = np.maximum(1e-5 * np.abs(ref_f), 5e-6 * 255) + ulp
+        bad = ~(np.abs(g.astype(np.float64) - r) <= tol)
+        assert not bad.any(), (what, int(bad.sum()), np.argwhere(bad)[:3].tolist())
+
+
+def _pyr_run(dev, img, sigma, rule, ok, shift, in_off=0):
+    import torch
+    import tucuda_amd
+    nrow, ncol = img.shape
+    big = np.zeros((nrow, ncol + in_off), dtype=img.dtype)
+    big[:, in_off:] = img
+    x = to_dev(big, dev)[:, in_off:]
+    o = to_dev(np.full((nrow // 2 + 1, ncol // 2 + 2), sentinel(TYPES[ok]), TYPES[ok]), dev)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma, rule)
+    g.smoothSubsample(x, o, shift)
+    torch.cuda.synchronize()
+    got = to_host(o)
+    fill = np.full(1, sentinel(TYPES[ok]), TYPES[ok])[0]      # the sentinel as stored in the output type
+    assert (got[nrow // 2:, :] == fill).all() and (got[:, ncol // 2:] == fill).all()
+    return got[:nrow // 2, :ncol // 2]
+
+
+@pytest.mark.parametrize("sigma,rule", [(0.8, 0), (1.0, 0), (2.0, 0), (3.0, 0), (0.8, 1), (1.3, 1), (3.1, 1)])
+@pytest.mark.parametrize("shift", [True, False])
+@pytest.mark.parametrize("shape", [(97, 61), (300, 250), (37, 517), (9, 40)])
+def test_pyramid_step(dev, sigma, rule, shift, shape):
+    img = _input("f32", *shape, seed=36)
+    ref, ref_f = _pyr_expected(img, sigma, rule, "f32", shift)
+    _pyr_check(_pyr_run(dev, img, sigma, rule, "f32", shift), ref, ref_f, "f32",
+               f"pyramid sigma={sigma} rule={rule} shift={shift} {shape}")
+
+
+@pytest.mark.parametrize("ik", ["u8", "f32", "f16", "bf16"])
+@pytest.mark.parametrize("ok", ["u8", "f32", "f16", "bf16"])
+def test_pyramid_type_pairs(dev, ik, ok):
+    img = _input(ik, 130, 250, seed=37)
+    ref, ref_f = _pyr_expected(img, 2.0, 0, ok, True)
+    _pyr_check(_pyr_run(dev, img, 2.0, 0, ok, True, in_off=1 if ik != "u8" else 3), ref, ref_f, ok,
+               f"pyramid {ik}->{ok}")
+
+
+@pytest.mark.parametrize("sigma", [1.0, 3.0])
+def test_pyramid_1080p_equals_two_step(dev, sigma):
+    """At full size: the fused step against smooth + subsample on the GPU
+    (same lobes, both paths FP32) and against the oracle."""
+    import torch
+    import tucuda_amd
+    img = _input("f32", 1080, 1920, seed=38)
+    x = to_dev(img, dev)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    full = torch.zeros_like(x)
+    g.smooth(x, full)
+    two = torch.zeros((540, 960), dtype=torch.float32, device=dev)
+    tucuda_amd.subsample(full, two, "identity1x1")
+    one = torch.full((540, 960), float("nan"), dtype=torch.float32, device=dev)
+    g.smoothSubsample(x, one)
+    torch.cuda.synchronize()
+    ref, ref_f = _pyr_expected(img, sigma, 0, "f32", True)
+    _pyr_check(to_host(one), ref, ref_f, "f32", f"pyramid 1080p sigma={sigma}")
+    _pyr_check(to_host(one), to_host(two), to_host(two), "f32", f"pyramid fused vs two-step sigma={sigma}")

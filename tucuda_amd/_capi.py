@@ -31,7 +31,7 @@ EXPORTS = (
     "tucu_last_error", "tucu_version", "tucu_malloc", "tucu_free", "tucu_row_stride",
     "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
     "tucu_fir2_convolve_taps",
-    "tucu_lobe_size", "tucu_subsample", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
+    "tucu_lobe_size", "tucu_subsample", "tucu_gauss2_pyramid", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
     "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
     "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
@@ -172,6 +172,7 @@ def load() -> ctypes.CDLL:
                                           ctypes.c_int, ctypes.c_void_p]
     L.tucu_lobe_size.argtypes = [fp, ctypes.c_int]
     L.tucu_subsample.argtypes = [imgp, imgp, ctypes.c_int, ctypes.c_void_p]
+    L.tucu_gauss2_pyramid.argtypes = [fp, ctypes.c_int, imgp, imgp, ctypes.c_int, ctypes.c_void_p]
     L.tucu_gauss2_lobes.argtypes = [ctypes.c_float, ctypes.c_int, fp, ip, fp, ip, fp, ip]
     L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,
                                        imgp, imgp, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]

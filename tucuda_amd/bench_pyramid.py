@@ -9,7 +9,11 @@ Workloads (seeded synthetic images, inputs resident in HBM):
   * a four-level Gaussian pyramid of the 1920x1080 float32 image: at each
     level FIRGaussianConvolver2(1.0).smooth into a full-size buffer, then
     subsample(identity1x1) into the next level (the reference's pyramid
-    step, Testprog/subsampleTest/main.cu after a smooth).
+    step, Testprog/subsampleTest/main.cu after a smooth); and the same
+    pyramid with the fused step (smoothSubsample, one launch per level);
+  * one fused step on 1920x1080 float32 (sigma 1 and 3), 3840x2160 float32
+    (sigma 3) and 3840x2160 u_char (sigma 2) images; algorithmic bytes: the
+    input once plus the quarter-size float32 output once.
 Launches are replayed from a captured HIP graph over 4 rotating buffer sets.
 Algorithmic bytes of a subsample: the input rows the operator reads (every
 second row for the 1-row operators, all rows otherwise) once, plus the
@@ -109,6 +113,34 @@ def main(argv=None):
     res["pyramid"]["1080p_f32_sigma1_4levels"] = {
         "us": round(us, 2), "launches": 6,
         "levels": ["x".join(map(str, s[::-1])) for s in levels]}
+
+    # the same pyramid with the fused step: smoothSubsample, three launches
+    def pyramid_fused():
+        for lv, _ in sets:
+            for k in range(3):
+                g.smoothSubsample(lv[k], lv[k + 1])
+
+    us = _graph_us(pyramid_fused, nsets, dev)
+    res["pyramid"]["1080p_f32_sigma1_4levels_fused"] = {"us": round(us, 2), "launches": 3}
+
+    # one fused step on larger images: input read once, quarter-size output written once
+    for name, img, sigma in (("1080p_f32_sigma1", f1080, 1.0), ("1080p_f32_sigma3", f1080, 3.0),
+                             ("2160p_f32_sigma3", f4k, 3.0), ("2160p_u8_sigma2", u4k, 2.0)):
+        gs = tucuda_amd.FIRGaussianConvolver2(sigma)
+        nrow, ncol = img.shape
+        ins = [tucuda_amd.array2(nrow, ncol, img.dtype, dev) for _ in range(nsets)]
+        for x in ins:
+            x.copy_(img)
+        outs = [tucuda_amd.array2(nrow // 2, ncol // 2, torch.float32, dev) for _ in range(nsets)]
+
+        def step():
+            for x, o in zip(ins, outs):
+                gs.smoothSubsample(x, o)
+
+        us = _graph_us(step, nsets, dev)
+        nbytes = nrow * ncol * img.element_size() + (nrow // 2) * (ncol // 2) * 4
+        res["pyramid"]["step_" + name] = {"us": round(us, 2), "gb_s": round(nbytes / us / 1e3, 1),
+                                           "frac": round(nbytes / us / 1e3 / HBM_PEAK_GB_S, 3), "bytes": nbytes}
     print(json.dumps(res), flush=True)
 
 

@@ -27,6 +27,7 @@ KernelEntry g_gauss_table[5][NGTH][4][4];
 KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 KernelEntry g_fir_mfma_table[2][4][4];
 SubLaunchFn g_sub_table[NSUBOPS][4][4];
+PyrLaunchFn g_pyr_table[3][4][4];
 
 size_t lobe_size(const float lobe[], bool even);
 void gauss_lobes_reference(float, float*, int*, float*, int*, float*, int*);
@@ -566,10 +567,50 @@ int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* st
     a.nrow = nrow; a.ncol = ncol;
     a.onrow = nrow / 2; a.oncol = ncol / 2;
     a.vec = reinterpret_cast<uintptr_t>(in->data) % uintptr_t(4 * es) == 0 && stride_b % (4 * es) == 0;
+    a.tiles_y = (a.onrow + SUB_WAVES * SUB_R - 1) / (SUB_WAVES * SUB_R);
     a.ostride = int32_t(out->stride);
     a.out = reinterpret_cast<uintptr_t>(out->data);
     const SubLaunchFn fn = g_sub_table[op][in->dtype][out->dtype];
     if (!fn) return fail(TUCU_ERR_INVALID, "no subsample kernel for this operator / type combination");
+    const hipError_t e = fn(a, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
+}
+
+int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in, const tucu_image* out,
+                        int shift, void* stream)
+{
+    int rc;
+    const int r = n0 - 1;
+    if (r < 1 || r > 16) return fail(TUCU_ERR_UNSUPPORTED_LOBE, "lobe 0 length outside the supported range");
+    if (!lobe0) return fail(TUCU_ERR_INVALID, "null lobe");
+    if ((rc = check_image(in, "input", false))) return rc;
+    if ((rc = check_image(out, "output", false))) return rc;
+    const int nrow = in->nrow, ncol = in->ncol;
+    if (nrow < 2 || ncol < 2) return TUCU_OK;                       // algorithm.h:227-233
+    if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
+    if ((out->nrow > 0 && out->nrow < nrow / 2) || (out->ncol > 0 && out->ncol < ncol / 2))
+        return fail(TUCU_ERR_INVALID, "output: smaller than nrow/2 x ncol/2");
+    if (out->stride > INT32_MAX) return fail(TUCU_ERR_INVALID, "output row stride too large");
+    const int64_t es = int64_t(dtype_size(in->dtype));
+    const int64_t stride_b = in->stride * es;
+    if (int64_t(nrow) * stride_b > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+
+    PyrArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data;
+    a.in_stride_b = int32_t(stride_b);
+    a.in_bytes = uint32_t(int64_t(nrow - 1) * stride_b + ncol * es);
+    a.nrow = nrow; a.ncol = ncol;
+    a.onrow = nrow / 2; a.oncol = ncol / 2;
+    a.r = r;
+    a.o = shift ? 0 : r;
+    a.ostride = int32_t(out->stride);
+    a.tiles_y = (a.onrow + PYR_TH - 1) / PYR_TH;
+    a.out = reinterpret_cast<uintptr_t>(out->data);
+    centre_index(lobe0, r, true, a.c);
+    const int ri = r <= 4 ? 0 : r <= 8 ? 1 : 2;
+    const PyrLaunchFn fn = g_pyr_table[ri][in->dtype][out->dtype];
+    if (!fn) return fail(TUCU_ERR_INVALID, "no kernel for this type combination");
     const hipError_t e = fn(a, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
 }

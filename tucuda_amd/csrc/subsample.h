@@ -71,6 +71,7 @@ struct SubArgs {
     int32_t   onrow, oncol;     // output extent written: nrow/2, ncol/2
     int32_t   vec;              // 4-pixel input loads are aligned (data and stride)
     int32_t   ostride;          // output row stride, elements
+    int32_t   tiles_y;          // workgroups per column strip
     uintptr_t out;              // output row 0
 };
 
@@ -265,8 +266,12 @@ subsample_wave(const SubArgs& a, int g, int yo0)
 template <int OP, class IN, class OUT>
 __global__ __launch_bounds__(64 * SUB_WAVES) void subsample_kernel(SubArgs a)
 {
-    const int g = blockIdx.x * 64 + threadIdx.x;                  // owns output columns 2g, 2g+1
-    const int yo0 = (blockIdx.y * SUB_WAVES + threadIdx.y) * SUB_R;   // first output row of the wave
+    // Workgroups in column-strip order, one contiguous range per XCD, so
+    // that vertically adjacent ones (sharing stencil halo rows) meet in one L2.
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = wg / a.tiles_y, by = wg % a.tiles_y;
+    const int g = bx * 64 + threadIdx.x;                          // owns output columns 2g, 2g+1
+    const int yo0 = (by * SUB_WAVES + threadIdx.y) * SUB_R;       // first output row of the wave
     if (yo0 >= a.onrow) return;                                   // wave-uniform
     // Wave-uniform load form: aligned 4-pixel loads when the image allows
     // them, unless a group of the last input row would reach past the end of
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(64 * SUB_WAVES) void subsample_kernel(SubArgs a)
     // stored.
     constexpr int RY = sub_ry(OP);
     const int last_row = 2 * yo0 + (RY == 0 ? 2 * (SUB_R - 1) : 2 * (SUB_R - 1) + RY);
-    if (a.vec && (4 * (int(blockIdx.x) * 64 + 63) + 3 < a.ncol || last_row < a.nrow - 1))
+    if (a.vec && (4 * (bx * 64 + 63) + 3 < a.ncol || last_row < a.nrow - 1))
         subsample_wave<OP, true, IN, OUT>(a, g, yo0);
     else
         subsample_wave<OP, false, IN, OUT>(a, g, yo0);
@@ -284,12 +289,137 @@ __global__ __launch_bounds__(64 * SUB_WAVES) void subsample_kernel(SubArgs a)
 template <int OP, class IN, class OUT>
 hipError_t launch_subsample(const SubArgs& a, hipStream_t s)
 {
-    const dim3 grid((a.oncol + 127) / 128, (a.onrow + SUB_WAVES * SUB_R - 1) / (SUB_WAVES * SUB_R));
-    hipLaunchKernelGGL((subsample_kernel<OP, IN, OUT>), grid, dim3(64, SUB_WAVES), 0, s, a);
+    const int nwg = (a.oncol + 127) / 128 * a.tiles_y;
+    hipLaunchKernelGGL((subsample_kernel<OP, IN, OUT>), dim3(nwg), dim3(64, SUB_WAVES), 0, s, a);
     return hipGetLastError();
 }
 
 using SubLaunchFn = hipError_t (*)(const SubArgs&, hipStream_t);
 extern SubLaunchFn g_sub_table[NSUBOPS][4][4];     // [operator][input dtype][output dtype]
+
+// ------------------------------------------------------------------------
+// The pyramid step fused: FIRGaussianConvolver2::smooth into a zero-filled
+// image followed by subsample(identity1x1) (FIRGaussianConvolver.h:178-182,
+// algorithm.h:183-242), in one pass that computes the smoothed image only
+// where the subsample reads it:
+//
+//   half[y][x] = sum_{i,j} k[i] k[j] in[Yc + i][Xc + j],  |i|, |j| <= r,
+//   (Yc, Xc) = (2y, 2x) + (shift ? 0 : (r, r)),
+//
+// where the window lies inside the image, else 0 (the zero border of the
+// intermediate), for y < nrow/2, x < ncol/2.  k is the symmetric lobe
+// (centre-indexed, zero beyond r).  No counterpart in the reference, which
+// writes the full-size smoothed image and reads it back.
+//
+// A workgroup owns PYR_TH x PYR_TW output pixels.  Vertical pass: thread t
+// owns input column Xc(x0) - R + t and loads the 2*PYR_TH - 1 + 2R rows its
+// column needs straight into registers (each wave load reads 64 consecutive
+// pixels), then forms the vertical sums centred on the PYR_TH even rows into
+// LDS.  Horizontal pass: a thread takes 4 adjacent outputs, reading the
+// 2R + 7 sums their windows span as 16-byte LDS vectors (one 4-byte read
+// per tap and output, at a 2-float stride, made the LDS the bottleneck).  Sums run outer tap pairs first, as FIRFilter.h:66-176
+// does (vertical pass first here, horizontal first there: the results
+// differ by FP32 rounding only).
+constexpr int PYR_TH = 16;      // output rows per workgroup
+constexpr int PYR_TW = 112;     // output columns per workgroup: 2*111 + 2*16 + 1 <= 256 input columns
+constexpr int PYR_P  = 264;     // LDS row pitch, floats
+
+struct PyrArgs {
+    const void* in;
+    uint32_t  in_bytes;         // bytes from row 0 to the end of the last row
+    int32_t   in_stride_b;      // bytes between input rows
+    int32_t   nrow, ncol;       // input extent
+    int32_t   onrow, oncol;     // output extent written: nrow/2, ncol/2
+    int32_t   r;                // lobe radius
+    int32_t   o;                // centre offset: shift ? 0 : r
+    int32_t   ostride;          // output row stride, elements
+    int32_t   tiles_y;          // tiles per column strip
+    uintptr_t out;
+    float     c[KMAX];          // centre-indexed lobe, zero beyond r
+};
+
+template <int R> __device__ __forceinline__ float pyr_sum(const PyrArgs& a, const float* s)
+{
+    float acc = a.c[R] * (s[0] + s[2 * R]);
+#pragma unroll
+    for (int j = R - 1; j >= 1; --j) acc = __builtin_fmaf(a.c[j], s[R - j] + s[R + j], acc);
+    return __builtin_fmaf(a.c[0], s[R], acc);
+}
+
+template <int R, class IN, class OUT>
+__global__ __launch_bounds__(256) void pyramid_kernel(PyrArgs a)
+{
+    constexpr int NW = 2 * (PYR_TH - 1) + 2 * R + 1;   // input rows per column
+    constexpr int NC = 2 * (PYR_TW - 1) + 2 * R + 1;   // input columns per tile
+    static_assert(NC <= 256, "tile wider than the workgroup");
+    __shared__ float V[PYR_TH][PYR_P];
+    const int t = threadIdx.x;
+    // Tiles in column-strip order, cut into one contiguous range per XCD:
+    // vertically adjacent tiles, which read the same 2R halo rows, run at
+    // the same time on one L2 (dispatch order alone spreads them over the
+    // eight XCDs and their L2s, and the halo rows come from HBM twice).
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int y0 = (tile % a.tiles_y) * PYR_TH, x0 = (tile / a.tiles_y) * PYR_TW;
+    const int yc0 = 2 * y0 + a.o, xc0 = 2 * x0 + a.o;     // centre of output (y0, x0)
+
+    if (t < NC) {
+        const __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
+        const int voff = (xc0 - R + t) * int(sizeof(IN));  // outside the image: 0 or a neighbouring row,
+        float w[NW];                                       // feeding only outputs that are zeroed
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int row = min(max(yc0 - R + i, 0), a.nrow - 1);
+            w[i] = Px<IN>::ld(rsrc, voff, row * a.in_stride_b);
+        }
+#pragma unroll
+        for (int k = 0; k < PYR_TH; ++k) V[k][t] = pyr_sum<R>(a, &w[2 * k]);
+    }
+    __syncthreads();
+
+    // Horizontal pass: a thread forms 4 adjacent outputs from the 2R + 7
+    // vertical sums their windows span, read as 16-byte LDS vectors.
+    constexpr int NQ = PYR_TW / 4;                        // 4-output groups per row
+    constexpr int NS = 2 * R + 8;
+    for (int gi = t; gi < PYR_TH * NQ; gi += 256) {
+        const int k = gi / NQ, xl = 4 * (gi % NQ);
+        const int y = y0 + k, x = x0 + xl;
+        if (y >= a.onrow || x >= a.oncol) continue;
+        const int yc = yc0 + 2 * k;
+        const bool yin = yc >= a.r && yc + a.r < a.nrow;
+        float s[NS];
+#pragma unroll
+        for (int m = 0; m < NS / 4; ++m) {
+            const v4f32 q = *reinterpret_cast<const v4f32*>(&V[k][2 * xl + 4 * m]);
+            s[4 * m] = q[0]; s[4 * m + 1] = q[1]; s[4 * m + 2] = q[2]; s[4 * m + 3] = q[3];
+        }
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int xc = xc0 + 2 * (xl + i);
+            v[i] = yin && xc >= a.r && xc + a.r < a.ncol ? pyr_sum<R>(a, &s[2 * i]) : 0.f;
+        }
+        const gptr<OUT> q = reinterpret_cast<gptr<OUT>>(a.out + uintptr_t(int64_t(y) * a.ostride + x) * sizeof(OUT));
+        if (x + 3 < a.oncol && (reinterpret_cast<uintptr_t>(q) & (4 * sizeof(OUT) - 1)) == 0) {
+            Px<OUT>::st4(q, v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (x + i < a.oncol) Px<OUT>::st(q + i, v[i]);
+        }
+    }
+}
+
+template <int R, class IN, class OUT>
+hipError_t launch_pyramid(const PyrArgs& a, hipStream_t s)
+{
+    const int tiles = (a.oncol + PYR_TW - 1) / PYR_TW * a.tiles_y;
+    hipLaunchKernelGGL((pyramid_kernel<R, IN, OUT>), dim3(tiles), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+constexpr int PYR_RADII[3] = {4, 8, 16};
+using PyrLaunchFn = hipError_t (*)(const PyrArgs&, hipStream_t);
+extern PyrLaunchFn g_pyr_table[3][4][4];           // [radius class][input dtype][output dtype]
 
 }   // namespace tucu

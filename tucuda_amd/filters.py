@@ -237,6 +237,17 @@ class FIRGaussianConvolver2(FIRFilter2):
     def diffVV(self, inp, out, shift: bool = True):   # :238-242
         self._method("diffVV", inp, out, shift)
 
+    def smoothSubsample(self, inp: torch.Tensor, out: torch.Tensor, shift: bool = True) -> None:
+        """Extension: the pyramid step in one launch.  ``out`` (at least
+        nrow/2 x ncol/2) receives what ``smooth`` into a zero-filled image
+        followed by ``subsample(..., "identity1x1")`` gives, without the
+        full-size intermediate."""
+        if inp.device != out.device:
+            raise ValueError("input and output must be on the same device")
+        src, dst = image_desc(inp), image_desc(out)
+        check(_capi.load().tucu_gauss2_pyramid(_fptr(self._lobe0), self._lobe0.size, ctypes.byref(src),
+                                               ctypes.byref(dst), int(bool(shift)), _stream_handle(inp)))
+
     def convolve6(self, inp: torch.Tensor, outs: OutputSpec, shift: bool = True) -> None:
         """Extension: every requested method from one fused launch.  ``outs``
         maps method names to output tensors, or lists up to six tensors in
