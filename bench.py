@@ -36,10 +36,11 @@ def algorithmic_bytes(nrow: int, ncol: int, radii, in_size: int = 4, out_size: i
     return total
 
 
-def cpu_baseline(img, sigma: float, budget_s: float = 12.0):
+def cpu_baseline(img, sigma: float, budget_s: float = 12.0, budget_1core_s: float = 4.0):
     """The oracle's FP32 restatement (H pass then V pass per output, FP32
     intermediate, no FMA contraction) on the host cores, over whole images,
-    as many images as fit in about ``budget_s`` seconds (at least one)."""
+    as many images as fit in about ``budget_s`` seconds (at least one); then
+    on one core for about ``budget_1core_s`` seconds."""
     import oracle
     import numpy as np
     try:
@@ -50,21 +51,54 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0):
     lobes = oracle.gauss_lobes(sigma)
     outs = {m: np.zeros(img.shape, np.float32) for m in oracle.METHOD_ORDER}
 
-    def one():
-        for m in oracle.METHOD_ORDER:
-            oracle.gauss(m, sigma, img, outs[m], lobes=lobes, nthreads=cores)
+    def rate(nthreads, budget):
+        def one():
+            for m in oracle.METHOD_ORDER:
+                oracle.gauss(m, sigma, img, outs[m], lobes=lobes, nthreads=nthreads)
 
-    one()                                             # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        one()
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or (n >= 3 and el * (n + 1) / n > 2 * budget_s):
-            break
-    gpix = img.size * n / el / 1e9
-    return {"value": round(gpix, 5), "unit": "Gpix/s", "cores": cores, "kind": "port",
-            "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)"}
+        one()                                         # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            one()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or (n >= 3 and el * (n + 1) / n > 2 * budget):
+                return n, el
+
+    n, el = rate(cores, budget_s)
+    n1, el1 = rate(1, budget_1core_s)
+    return {"value": round(img.size * n / el / 1e9, 5), "unit": "Gpix/s", "cores": cores, "kind": "port",
+            "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)",
+            "value_1core": round(img.size * n1 / el1 / 1e9, 5),
+            "sample_1core": f"{n1} whole images on 1 thread ({el1:.1f} s)"}
+
+
+def hot_kernel_us(g, ins, outs, launches: int = 10, reps: int = 20) -> float:
+    """Microseconds per launch with one buffer set used by every launch (the
+    input and outputs stay in the 256 MB Infinity Cache between launches):
+    the cache-hot figure beside the rotating one."""
+    import torch
+    dev = ins.device
+    stream = torch.cuda.current_stream(dev)
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream(dev)
+    cap.wait_stream(stream)
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(graph, stream=cap):
+            for _ in range(launches):
+                g.convolve6(ins, outs)
+    stream.wait_stream(cap)
+    for _ in range(2):
+        graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        graph.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    del graph
+    return e0.elapsed_time(e1) * 1000 / (reps * launches)
 
 
 def write_fill_gbs(dev, nbytes: int, nbuf: int = 10, reps: int = 5) -> float:
@@ -357,6 +391,11 @@ def main():
             "algorithmic_bytes": alg_bytes,
         },
     }
+    # The cache-hot figure (one buffer set for every launch) beside the
+    # rotating one that `achieved` uses.
+    hot_us = hot_kernel_us(g, *sets[0])
+    line["roofline"]["kernel_us_hot_single_buffer"] = round(hot_us, 3)
+    log(f"cache-hot (one buffer set): {hot_us:.2f} us per launch")
     # Context for `frac`: the rate a plain fill of the six outputs' bytes
     # reaches (the outputs are 85% of the kernel's bytes).
     fill = write_fill_gbs(dev, 6 * nrow * ncol * 4)
