@@ -128,3 +128,56 @@ def test_fused_six_large_image(dev, sigma, ik):
     g.convolve6(to_dev(img, dev), outs)
     for m in oracle.METHOD_ORDER:
         assert_f32_parity(to_host(outs[m]), refs[m], f"large sigma={sigma} {ik} {m}")
+
+
+@pytest.mark.parametrize("k", [-20, -7, 9, 24])
+def test_power_of_two_scale_equivariance(dev, kernel_family, k):
+    """Scaling the image by 2^k scales every output by exactly 2^k (every
+    internal scale of either kernel family is a power of two), and the scaled
+    outputs meet the FP32 bar against the oracle with its absolute floor
+    scaled alike (5e-6 of the data range)."""
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(41, 200, 300)
+    scaled = (img.astype(np.float64) * 2.0 ** k).astype(np.float32)
+    g = tucuda_amd.FIRGaussianConvolver2(2.5)
+    outs = {}
+    for name, x in (("base", img), ("scaled", scaled)):
+        d = to_dev(x, dev)
+        o = {m: torch.zeros_like(d) for m in tucuda_amd.METHOD_ORDER}
+        g.convolve6(d, o)
+        torch.cuda.synchronize()
+        outs[name] = {m: to_host(t) for m, t in o.items()}
+    lobes = oracle.gauss_lobes(2.5)
+    for m in tucuda_amd.METHOD_ORDER:
+        want = (outs["base"][m].astype(np.float64) * 2.0 ** k).astype(np.float32)
+        np.testing.assert_array_equal(outs["scaled"][m], want, err_msg=f"{m} 2^{k}")
+        ref = np.zeros(img.shape, np.float32)
+        oracle.gauss(m, 2.5, scaled, ref, lobes=lobes)
+        err = np.abs(outs["scaled"][m].astype(np.float64) - ref)
+        tol = np.maximum(1e-5 * np.abs(ref), 5e-6 * 255 * 2.0 ** k)
+        assert (err <= tol).all(), (m, k, float((err / tol).max()))
+
+
+@pytest.mark.parametrize("spike", [1e4, 1e6])
+def test_high_dynamic_range(dev, kernel_family, spike):
+    """A 0..767 image with sparse pixels of 1e4 or 1e6: every output meets
+    the FP32 bar of the [0,255] data.  (The matrix-core kernel scales each
+    strip by its largest input; with its taps scaled to 2^11..2^12 the input
+    split's low halves fell below f16's normal range and a fifth of the
+    pixels near the 1e6 values missed the bar.)"""
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(52, 270, 480)
+    rng = np.random.default_rng(4)
+    img[rng.integers(0, 270, 12), rng.integers(0, 480, 12)] = spike
+    g = tucuda_amd.FIRGaussianConvolver2(3.0)
+    d = to_dev(img, dev)
+    outs = {m: torch.zeros_like(d) for m in tucuda_amd.METHOD_ORDER}
+    g.convolve6(d, outs)
+    torch.cuda.synchronize()
+    lobes = oracle.gauss_lobes(3.0)
+    for m in tucuda_amd.METHOD_ORDER:
+        ref = np.zeros(img.shape, np.float32)
+        oracle.gauss(m, 3.0, img, ref, lobes=lobes)
+        assert_f32_parity(to_host(outs[m]), ref, f"{m} spikes of {spike:g}")

@@ -110,7 +110,7 @@ int launch_fir_mfma(SepArgs& a, const float* vt, const float* ht, int rV, int rH
         for (int i = 0; i < TMAX; ++i) { cmax = std::max(cmax, std::fabs(t[i])); l1 += std::fabs(t[i]); }
         int x = 0;
         (void)std::frexp(cmax, &x);             // cmax < 2^x
-        e[l] = 12 - x;
+        e[l] = MFMA_TAP_EXP - x;
         a.mp[l] = std::ldexp(1.f, e[l]);
         if (l == 0) (void)std::frexp(l1 * a.mp[0], &eq);   // l1 * mp < 2^eq
     }
@@ -170,7 +170,7 @@ void centre_index(const float* lobe, int r, bool symmetric, float c[KMAX])
 }
 
 // Power-of-two operand scales of the matrix-core kernels (mfma.h): taps of
-// lobe l times mp[l] peak in [2^11, 2^12); mq makes mq * mp[l] * (L1 norm of
+// lobe l times mp[l] peak in [2^5, 2^6); mq makes mq * mp[l] * (L1 norm of
 // lobe l) <= 1 for every lobe, so that fields of an input scaled below 2^13
 // stay below 2^13; mu[k] undoes the tap and input scales of output k = (H
 // lobe, V lobe).
@@ -187,7 +187,7 @@ void mfma_scales(SepArgs& a)
         }
         int e = 0;
         (void)std::frexp(cmax, &e);             // cmax < 2^e
-        ep[l] = 12 - e;
+        ep[l] = MFMA_TAP_EXP - e;
         a.mp[l] = std::ldexp(1.f, ep[l]);
         (void)std::frexp(l1 * a.mp[l], &e);     // l1 * mp < 2^e
         emax = std::max(emax, e);

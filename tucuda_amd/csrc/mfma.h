@@ -23,12 +23,18 @@
 // and split as hi = f16(v), lo = f16(v - hi); the products hi*hi + hi*lo +
 // lo*hi carry about 22 significant bits (the dropped lo*lo term and the
 // split residual are each below 2^-22 relative), summed in FP32.  Scales
-// (all powers of two): taps of lobe l by mp[l] (largest tap in [2^11,
-// 2^12)); the input of each strip by mq times a factor putting its largest
-// magnitude in [2^12, 2^13), mq <= 1 / (L1 norm * mp) of every lobe, so that
-// the vertical fields stay below 2^13 and no operand leaves the f16 range;
-// each output is multiplied back by mu[k] / (the strip's factor).  Every scale is a power of two, so apart from f16 subnormals
-// (values 2^-16 below a strip's largest) the results do not depend on them.
+// (all powers of two): taps of lobe l by mp[l] (largest tap in [2^5, 2^6),
+// MFMA_TAP_EXP); the input of each strip by mq times a factor putting its
+// largest magnitude in [2^12, 2^13), mq <= 1 / (L1 norm * mp) of every lobe,
+// so that the vertical fields stay below 2^13 and no operand leaves the f16
+// range; each output is multiplied back by mu[k] / (the strip's factor).
+// Where lo falls below f16's normal range (2^-14) it keeps an absolute
+// 2^-25 instead of 11 more bits, so the error of an output is bounded by
+// about 2^-25 (L1 / X + n tapmax / T) times the strip's largest input, with
+// X and T the scaled input and tap peaks; the tap peak of 2^6 balances the
+// two terms (a peak of 2^12 left the inputs at a quarter and their lo
+// halves subnormal below half the strip's largest value: 20x the error on
+// an image whose strips mix 0..255 data with values of 10^6).
 // Non-finite inputs make every output of their 16 x 16 block window
 // non-finite (the band products multiply them by zero).
 #pragma once
@@ -54,6 +60,7 @@ constexpr int MT4      = 112;             // tap-run table entries per lobe
 constexpr int MBASE    = 64;              // run t starts with tap t - MBASE
 constexpr int MHP      = MT4 - 16;        // entries of the horizontal pair table
 constexpr int MINVALID = int(0x80000000u);   // buffer offset past any image: loads return 0
+constexpr int MFMA_TAP_EXP = 6;           // largest scaled tap of each lobe in [2^5, 2^6)
 
 template <int RC> struct MGeo {
     static constexpr bool K48 = RC == 16;                 // K = 48 (two 32-entry slices), else 32
