@@ -102,6 +102,10 @@ main(int argc, char* argv[])
         cu::Array2<float> half(in_d.nrow() / 2, in_d.ncol() / 2);
         cu::subsample(outs[0].cbegin(), outs[0].cend(), half.begin(), cu::identity1x1());
         save(dir + "/half_smooth.pfm", half);
+        // the same step fused (extension): no full-size intermediate
+        cu::Array2<float> half_fused(in_d.nrow() / 2, in_d.ncol() / 2);
+        cudaFilter.smoothSubsample(in_d.cbegin(), in_d.cend(), half_fused.begin());
+        save(dir + "/half_smooth_fused.pfm", half_fused);
 
         // the six from one fused launch
         std::array<Out, 6> fused;

@@ -81,6 +81,7 @@ def test_dropin_demo(dev, tmp_path):
     smooth = read_pnm(tmp_path / "smooth_f32.pfm")
     half = read_pnm(tmp_path / "half_smooth.pfm")
     np.testing.assert_array_equal(half, smooth[0:150:2, 0:260:2])
+    assert_f32_parity(read_pnm(tmp_path / "half_smooth_fused.pfm"), half, "drop-in smoothSubsample")
     ref = np.zeros((75, 130), np.float32)
     oracle.subsample(img.astype(np.float32), ref, "gaussian5x5")
     np.testing.assert_array_equal(read_pnm(tmp_path / "gaussian5x5_half.pfm"), ref)
