@@ -84,3 +84,43 @@ def test_ceil4sigma_rule():
         dist = np.arange(r, 0, -1, dtype=np.float64)
         assert abs(2 * np.dot(l2[:-1].astype(np.float64), dist ** 2) - 2.0) < 1e-4
         assert (l1 < 0).all()
+
+
+def _img(data, stride, nrow, ncol, dt):
+    return _capi.TucuImage(data, stride, nrow, ncol, dt, 0)
+
+
+def test_subsample_and_pyramid_argument_checks():
+    """Argument checks of tucu_subsample and tucu_gauss2_pyramid, which
+    return before any device work: unknown operator, integer input to an
+    arithmetic operator, output too small, too-small input (a no-op,
+    algorithm.h:227-233), misaligned data, unsupported lobe length."""
+    lib = _capi.load()
+    fake = 0x10000                                   # never dereferenced on these paths
+    f32, u8 = _capi.TUCU_F32, _capi.TUCU_U8
+    src = _img(fake, 64, 32, 64, f32)
+    dst = _img(fake, 32, 16, 32, f32)
+    assert lib.tucu_subsample(ctypes.byref(src), ctypes.byref(dst), 12, None) == _capi.TUCU_ERR_INVALID
+    assert b"unknown subsample operator" in lib.tucu_last_error()
+    assert lib.tucu_subsample(ctypes.byref(src), ctypes.byref(dst), -1, None) == _capi.TUCU_ERR_INVALID
+    src8 = _img(fake, 64, 32, 64, u8)
+    assert lib.tucu_subsample(ctypes.byref(src8), ctypes.byref(dst), 1, None) == _capi.TUCU_ERR_INVALID
+    assert b"floating-point" in lib.tucu_last_error()
+    small = _img(fake, 32, 15, 32, f32)
+    assert lib.tucu_subsample(ctypes.byref(src), ctypes.byref(small), 0, None) == _capi.TUCU_ERR_INVALID
+    assert b"smaller" in lib.tucu_last_error()
+    one_row = _img(fake, 64, 1, 64, f32)
+    assert lib.tucu_subsample(ctypes.byref(one_row), ctypes.byref(dst), 11, None) == _capi.TUCU_OK
+    odd = _img(fake + 2, 64, 32, 64, f32)
+    assert lib.tucu_subsample(ctypes.byref(odd), ctypes.byref(dst), 0, None) == _capi.TUCU_ERR_INVALID
+    assert b"aligned" in lib.tucu_last_error()
+
+    lobe = np.array([0.1, 0.2, 0.4], np.float32)
+    fp = lobe.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    assert lib.tucu_gauss2_pyramid(fp, 1, ctypes.byref(src), ctypes.byref(dst), 1, None) == \
+        _capi.TUCU_ERR_UNSUPPORTED_LOBE
+    assert lib.tucu_gauss2_pyramid(fp, 18, ctypes.byref(src), ctypes.byref(dst), 1, None) == \
+        _capi.TUCU_ERR_UNSUPPORTED_LOBE
+    assert lib.tucu_gauss2_pyramid(fp, 3, ctypes.byref(src), ctypes.byref(small), 1, None) == _capi.TUCU_ERR_INVALID
+    assert lib.tucu_gauss2_pyramid(fp, 3, ctypes.byref(one_row), ctypes.byref(dst), 1, None) == _capi.TUCU_OK
+    assert lib.tucu_gauss2_pyramid(None, 3, ctypes.byref(src), ctypes.byref(dst), 1, None) == _capi.TUCU_ERR_INVALID
