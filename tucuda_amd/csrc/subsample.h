@@ -320,7 +320,7 @@ extern SubLaunchFn g_sub_table[NSUBOPS][4][4];     // [operator][input dtype][ou
 // per tap and output, at a 2-float stride, made the LDS the bottleneck).  Sums run outer tap pairs first, as FIRFilter.h:66-176
 // does (vertical pass first here, horizontal first there: the results
 // differ by FP32 rounding only).
-constexpr int PYR_TH = 16;      // output rows per workgroup
+constexpr int PYR_TH = 8;       // output rows per workgroup (16 and 32 measured slower: fewer workgroups)
 constexpr int PYR_TW = 112;     // output columns per workgroup: 2*111 + 2*16 + 1 <= 256 input columns
 constexpr int PYR_P  = 264;     // LDS row pitch, floats
 
