@@ -305,6 +305,8 @@ TUCU_API int    tucu_server_submit(tucu_server server, const tucu_request* req, 
 TUCU_API int    tucu_server_submit_many(tucu_server server, const tucu_request* reqs, int n, uint64_t* ids);
 /* Moves up to max completions into out; *n = number moved. */
 TUCU_API int    tucu_server_poll(tucu_server server, tucu_completion* out, int max, int* n);
+/* Hands the outputs of a completed request back to the pool; a request not
+ * yet completed (queued or in flight) is refused with TUCU_ERR_INVALID. */
 TUCU_API int    tucu_server_release(tucu_server server, uint64_t id);
 TUCU_API int    tucu_server_stats(tucu_server server, tucu_server_counters* stats);
 

@@ -210,6 +210,29 @@ def test_server_errors(dev):
             srv.release(rid)                                  # already released
 
 
+def test_release_refuses_unfinished(dev):
+    """Releasing a request that has not completed (here: queued behind the
+    pool, which a completed request still holds) is refused and leaves the
+    request intact; it completes normally once memory frees up."""
+    import torch
+    import tucuda_amd
+    from tucuda_amd.serve import Server
+    x = torch.zeros((256, 512), dtype=torch.bfloat16, device=dev)
+    with Server(1 << 20) as srv:                              # 1 MiB of outputs
+        ra = srv.submit(x, 2.0, 0x03)                         # 2 outputs: 512 KiB
+        _wait(srv, [ra])
+        rb = srv.submit(x, 2.0, 0x07)                         # 3 outputs: 768 KiB, waits for ra's memory
+        time.sleep(0.05)
+        assert srv.poll() == []
+        with pytest.raises(tucuda_amd.TucuError, match="not completed"):
+            srv.release(rb)
+        srv.release(ra)
+        cb = _wait(srv, [rb])[rb]
+        assert cb["status"] == 0
+        srv.release(rb)
+        assert srv.stats()["pool_used"] == 0
+
+
 def test_submit_many(dev):
     """Requests submitted in one call run like ones submitted one by one
     (same outputs as their single-image launches, ids in order), and a call

@@ -95,6 +95,8 @@ struct Request {
     size_t       off[6] = {0}, bytes[6] = {0};
     tucu_image   out[6];
     double       admitted = 0, launched = 0;
+    bool         done = false;     // completed (guarded by the server's mu); until then the
+                                   // scheduler owns the request and may not see it released
 };
 
 struct Slot {
@@ -185,6 +187,7 @@ void complete(tucu_server_s& s, Request* r, int status, double t)
     }
     std::lock_guard<std::mutex> lock(s.mu);
     s.completed.push_back(c);
+    r->done = true;                    // the last touch of r by the scheduler
     s.done_reqs++;
 }
 
@@ -491,6 +494,9 @@ int tucu_server_release(tucu_server server, uint64_t id)
         std::lock_guard<std::mutex> lock(server->mu);
         auto it = server->live.find(id);
         if (it == server->live.end()) return fail(TUCU_ERR_INVALID, "unknown or already released request");
+        // Queued or in flight: the scheduler still holds the request and a
+        // launch may still write its outputs.
+        if (!it->second->done) return fail(TUCU_ERR_INVALID, "request not completed yet");
         r = it->second;
         server->live.erase(it);
     }
