@@ -260,9 +260,12 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
     if (batch->jobs.empty()) return TUCU_OK;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     reps = std::max(1, reps);
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        return device_fail(hipGetLastError(), "hipEventCreate");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        const hipError_t err = hipGetLastError();
+        if (e0) (void)hipEventDestroy(e0);
+        return device_fail(err, "hipEventCreate");
+    }
     int best = -1;
     float best_t = 0.f;
     int rc = TUCU_OK;
@@ -285,6 +288,7 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    if (best < 0) return fail(TUCU_ERR_INVALID, "no batched kernel for this type pair");
     batch->th_index = -1;
     batch->active = best;
     {
