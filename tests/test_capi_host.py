@@ -124,3 +124,14 @@ def test_subsample_and_pyramid_argument_checks():
     assert lib.tucu_gauss2_pyramid(fp, 3, ctypes.byref(src), ctypes.byref(small), 1, None) == _capi.TUCU_ERR_INVALID
     assert lib.tucu_gauss2_pyramid(fp, 3, ctypes.byref(one_row), ctypes.byref(dst), 1, None) == _capi.TUCU_OK
     assert lib.tucu_gauss2_pyramid(None, 3, ctypes.byref(src), ctypes.byref(dst), 1, None) == _capi.TUCU_ERR_INVALID
+
+
+def test_memcpy2d_argument_checks():
+    lib = _capi.load()
+    buf = ctypes.create_string_buffer(64)
+    assert lib.tucu_memcpy2d(buf, 16, buf, 16, 8, 2, 3) == _capi.TUCU_ERR_INVALID
+    assert b"copy kind" in lib.tucu_last_error()
+    assert lib.tucu_memcpy2d(buf, 4, buf, 16, 8, 2, 0) == _capi.TUCU_ERR_INVALID
+    assert b"pitch" in lib.tucu_last_error()
+    assert lib.tucu_memcpy2d(None, 16, buf, 16, 8, 2, 0) == _capi.TUCU_ERR_INVALID
+    assert lib.tucu_memcpy2d(buf, 16, buf, 16, 0, 2, 0) == _capi.TUCU_OK      # nothing to copy

@@ -391,8 +391,10 @@ size_t tucu_row_stride(size_t ncol, size_t elem_size)
 int tucu_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch,
                   size_t width_bytes, size_t height, int kind)
 {
+    if (kind < 0 || kind > 2) return fail(TUCU_ERR_INVALID, "copy kind must be 0, 1 or 2");
     if (width_bytes == 0 || height == 0) return TUCU_OK;
     if (!dst || !src) return fail(TUCU_ERR_INVALID, "null pointer");
+    if (dpitch < width_bytes || spitch < width_bytes) return fail(TUCU_ERR_INVALID, "pitch shorter than a row");
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
                           : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     const hipError_t e = hipMemcpy2D(dst, dpitch, src, spitch, width_bytes, height, k);
