@@ -241,3 +241,41 @@ def test_pyramid_1080p_equals_two_step(dev, sigma):
     ref, ref_f = _pyr_expected(img, sigma, 0, "f32", True)
     _pyr_check(to_host(one), ref, ref_f, "f32", f"pyramid 1080p sigma={sigma}")
     _pyr_check(to_host(one), to_host(two), to_host(two), "f32", f"pyramid fused vs two-step sigma={sigma}")
+
+
+def test_graph_replays_equal_eager(dev):
+    """subsample and the fused pyramid step captured in a HIP graph: every
+    replay (with eager work and synchronisations between them, the inputs
+    changed between replays) equals the eager launches bit for bit."""
+    import torch
+    import tucuda_amd
+    g = tucuda_amd.FIRGaussianConvolver2(2.0)
+    x = to_dev(_input("f32", 301, 403, seed=39), dev)
+    half_a = torch.zeros((150, 201), dtype=torch.float32, device=dev)
+    half_b = torch.zeros_like(half_a)
+
+    def work():
+        tucuda_amd.subsample(x, half_a, "gaussian5x5")
+        g.smoothSubsample(x, half_b)
+
+    work()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    st = torch.cuda.Stream(dev)
+    st.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(st):
+        with torch.cuda.graph(graph, stream=st):
+            work()
+    torch.cuda.current_stream(dev).wait_stream(st)
+    for rep in range(4):
+        x.copy_(to_dev(_input("f32", 301, 403, seed=40 + rep), dev))
+        half_a.fill_(float("nan"))
+        half_b.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got_a, got_b = to_host(half_a), to_host(half_b)
+        work()                                                # eager, same inputs
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got_a, to_host(half_a), err_msg=f"subsample replay {rep}")
+        np.testing.assert_array_equal(got_b, to_host(half_b), err_msg=f"pyramid replay {rep}")
