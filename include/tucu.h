@@ -135,7 +135,8 @@ TUCU_API int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* ta
 /* ---- FIRGaussianConvolver2 ---------------------------------------------- */
 
 /* detail::lobeSize(lobe, even) (FIRGaussianConvolver.cu:52-98) over a
- * 17-element lobe. */
+ * 17-element lobe: returns the size (2..17), or -TUCU_ERR_INVALID for a
+ * null lobe (a size, not a tucu_status, so errors are negative). */
 TUCU_API int tucu_lobe_size(const float* lobe17, int even);
 
 /* FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)

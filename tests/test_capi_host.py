@@ -135,3 +135,9 @@ def test_memcpy2d_argument_checks():
     assert b"pitch" in lib.tucu_last_error()
     assert lib.tucu_memcpy2d(None, 16, buf, 16, 8, 2, 0) == _capi.TUCU_ERR_INVALID
     assert lib.tucu_memcpy2d(buf, 16, buf, 16, 0, 2, 0) == _capi.TUCU_OK      # nothing to copy
+
+
+def test_lobe_size_null_is_negative():
+    lib = _capi.load()
+    assert lib.tucu_lobe_size(None, 0) == -_capi.TUCU_ERR_INVALID
+    assert b"null lobe" in lib.tucu_last_error()

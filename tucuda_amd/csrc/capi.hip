@@ -501,7 +501,7 @@ int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int 
 
 int tucu_lobe_size(const float* lobe17, int even)
 {
-    if (!lobe17) return fail(TUCU_ERR_INVALID, "null lobe");
+    if (!lobe17) { fail(TUCU_ERR_INVALID, "null lobe"); return -TUCU_ERR_INVALID; }
     return int(lobe_size(lobe17, even != 0));
 }
 
