@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <initializer_list>
 #include <cstdint>
+#include <cstring>
 #include <istream>
 #include <ostream>
 #include <stdexcept>
@@ -79,8 +80,11 @@ class Array2
     std::vector<T> _v;
 };
 
-//! Grey-level host image with PNM I/O (8-bit P5 / P6 in, P5 out; float
-//! images as binary "PF" grey-level PFM).
+//! Grey-level host image with PNM I/O: 8-bit P5 / P6 and float Pf / PF
+//! (PFM) in, P5 (byte images) or Pf (others) out.  Header comment lines
+//! (TULibs' EPBM keys such as "# DataType: Char") are skipped, and a stream
+//! may hold several images one after another, as the reference's
+//! Testprog/ueshiba.epbm does: each restore() reads the next one.
 template <class T>
 class Image : public Array2<T>
 {
@@ -93,7 +97,8 @@ class Image : public Array2<T>
     size_t height() const { return this->nrow(); }
 
     //! Reads a binary PNM (P5 grey, P6 colour reduced to grey by the mean
-    //! of the channels, PF/Pf float).
+    //! of the channels; Pf grey float, PF colour float reduced likewise,
+    //! byte order from the sign of the scale: negative little-endian).
     std::istream& restore(std::istream& in)
     {
         std::string magic;
@@ -121,15 +126,29 @@ class Image : public Array2<T>
                 for (size_t c = 0; c < ch; ++c) s += buf[i * ch + c];
                 this->data()[i] = T(s / ch);
             }
-        } else if (magic == "Pf") {
+        } else if (magic == "Pf" || magic == "PF") {
             double scale;
             skip(); in >> scale;
             in.get();
-            std::vector<float> buf(w * h);
-            in.read(reinterpret_cast<char*>(buf.data()), std::streamsize(buf.size() * sizeof(float)));
+            const size_t ch = magic == "PF" ? 3 : 1;
+            std::vector<uint32_t> buf(w * h * ch);
+            in.read(reinterpret_cast<char*>(buf.data()), std::streamsize(buf.size() * sizeof(uint32_t)));
+            const uint32_t one = 1;
+            const bool host_little = *reinterpret_cast<const unsigned char*>(&one) == 1;
+            if ((scale < 0) != host_little)             // file byte order differs from the host's
+                for (uint32_t& v : buf)
+                    v = (v >> 24) | ((v >> 8) & 0xff00u) | ((v << 8) & 0xff0000u) | (v << 24);
             this->resize(h, w);
             for (size_t y = 0; y < h; ++y)              // PFM stores the bottom row first
-                for (size_t x = 0; x < w; ++x) (*this)[h - 1 - y][x] = T(buf[y * w + x]);
+                for (size_t x = 0; x < w; ++x) {
+                    float s = 0.f;
+                    for (size_t c = 0; c < ch; ++c) {
+                        float f;
+                        std::memcpy(&f, &buf[(y * w + x) * ch + c], sizeof(f));
+                        s += f;
+                    }
+                    (*this)[h - 1 - y][x] = T(s / float(ch));
+                }
         } else {
             throw std::runtime_error("Image::restore: unsupported PNM type " + magic);
         }
