@@ -141,6 +141,8 @@ class SeparableFilter2:
         return self._tv.size // 2
 
     def convolve(self, inp: torch.Tensor, out: torch.Tensor, shift: bool = True) -> None:
+        if inp.device != out.device:
+            raise ValueError("input and output must be on the same device")
         src, dst = image_desc(inp), image_desc(out)
         check(_capi.load().tucu_fir2_convolve_taps(
             _fptr(self._th), self._th.size, _fptr(self._tv), self._tv.size,
