@@ -71,6 +71,10 @@ def lib() -> ctypes.CDLL:
         L.orc_gauss_lobes_f32.restype = None
         L.orc_gauss_lobes_f64.argtypes = [ctypes.c_double, f64p, ip, f64p, ip, f64p, ip]
         L.orc_gauss_lobes_f64.restype = None
+        L.orc_gauss_lobes_ceil4_f32.argtypes = [ctypes.c_float, f32p, ip, f32p, ip, f32p, ip]
+        L.orc_gauss_lobes_ceil4_f32.restype = None
+        L.orc_gauss_lobes_ceil4_f64.argtypes = [ctypes.c_double, f64p, ip, f64p, ip, f64p, ip]
+        L.orc_gauss_lobes_ceil4_f64.restype = None
         common = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                   ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.orc_fir2_f32.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_int] + common
@@ -116,13 +120,23 @@ def lobe_size(lobe17, even: bool, precision: str = "f32") -> int:
     return lib().orc_lobe_size_f64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(even))
 
 
-def gauss_lobes(sigma: float, precision: str = "f32"):
+def gauss_lobes(sigma: float, precision: str = "f32", rule: str = "reference"):
     """FIRGaussianConvolver2::initialize(sigma, lobe0, lobe1, lobe2)
-    (FIRGaussianConvolver.h:123-170).  Returns three numpy arrays."""
+    (FIRGaussianConvolver.h:123-170).  Returns three numpy arrays.
+
+    ``rule`` "reference" sizes the lobes with detail::lobeSize
+    (FIRGaussianConvolver.cu:52-98); "ceil4" fixes the radius at
+    r = ceil(4 sigma), clamped to [1, 16] (lobes of r + 1, r, r + 1 entries),
+    with the same normalisation formulas."""
+    if rule not in ("reference", "ceil4"):
+        raise ValueError(f"unknown lobe rule {rule!r}")
+    c4 = rule == "ceil4"
     if precision == "f32":
-        dt, ct, fn = np.float32, ctypes.c_float, lib().orc_gauss_lobes_f32
+        dt, ct = np.float32, ctypes.c_float
+        fn = lib().orc_gauss_lobes_ceil4_f32 if c4 else lib().orc_gauss_lobes_f32
     else:
-        dt, ct, fn = np.float64, ctypes.c_double, lib().orc_gauss_lobes_f64
+        dt, ct = np.float64, ctypes.c_double
+        fn = lib().orc_gauss_lobes_ceil4_f64 if c4 else lib().orc_gauss_lobes_f64
     bufs = [np.zeros(LOBE_SIZE_MAX, dtype=dt) for _ in range(3)]
     ns = [ctypes.c_int(0) for _ in range(3)]
     p = ctypes.POINTER(ct)
@@ -189,13 +203,44 @@ def fir2_taps(tapsH, tapsV, img: np.ndarray, out: np.ndarray, shift: bool = True
 
 
 def gauss(method: str, sigma: float, img: np.ndarray, out: np.ndarray, shift: bool = True,
-          precision: str = "f32", nthreads: int = 1, lobes=None) -> np.ndarray:
+          precision: str = "f32", nthreads: int = 1, lobes=None, rule: str = "reference") -> np.ndarray:
     """FIRGaussianConvolver2(sigma).<method>(img, out, shift)
-    (FIRGaussianConvolver.h:178-242)."""
+    (FIRGaussianConvolver.h:178-242).  ``lobes`` overrides the lobes
+    ``gauss_lobes(sigma, precision, rule)`` would give."""
     if lobes is None:
-        lobes = gauss_lobes(sigma, precision)
+        lobes = gauss_lobes(sigma, precision, rule)
     h, v = METHODS[method]
     return fir2(lobes[h], lobes[v], img, out, shift, precision, nthreads)
+
+
+def role_taps(lobe, symmetric: bool, precision: str = "f32") -> np.ndarray:
+    """The 2r+1 taps of a Gaussian lobe whose parity comes from its role, not
+    its length: lobe 0 and lobe 2 symmetric (r + 1 entries, centre last),
+    lobe 1 antisymmetric (r entries, the left / upper side weighted +c[k],
+    the other side -c[k], zero centre), lobe[0] outermost as
+    FIRFilter.h:66-176 orders them."""
+    lobe = np.asarray(lobe, np.float64)
+    r = lobe.size - 1 if symmetric else lobe.size
+    k = np.zeros(2 * r + 1)
+    for i in range(1, r + 1):
+        c = lobe[r - i]
+        k[r - i] = c
+        k[r + i] = c if symmetric else -c
+    if symmetric:
+        k[r] = lobe[r]
+    return k.astype(np.float32 if precision == "f32" else np.float64)
+
+
+def gauss_general(method: str, lobes, img: np.ndarray, out: np.ndarray, shift: bool = True,
+                  precision: str = "f32", nthreads: int = 1) -> np.ndarray:
+    """One of the six methods (FIRGaussianConvolver.h:178-242, the (H, V)
+    lobe pairs of METHODS) for lobes of any length, e.g. those of the
+    ceil(4 sigma) rule, whose symmetric lobes have an even number of entries
+    when r is odd (FIRFilter2 would take such a lobe for an antisymmetric
+    one).  Computed as general taps (``fir2_taps``), left tap first."""
+    h, v = METHODS[method]
+    return fir2_taps(role_taps(lobes[h], h != 1, precision), role_taps(lobes[v], v != 1, precision),
+                     img, out, shift, precision, nthreads)
 
 
 def subsample(img: np.ndarray, out: np.ndarray, op) -> np.ndarray:

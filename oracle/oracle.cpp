@@ -19,6 +19,7 @@
 //   TU/cu/FIRFilter.h:286-287    shifted output slice
 //   TU/cu/FIRFilter.h:298-377    early return, intermediate type, size dispatch
 //   TU/cu/FIRGaussianConvolver.h:123-170  Gaussian lobes and their normalisation
+//                                         (also applied to the ceil(4 sigma) radius rule)
 //   TU/cu/FIRGaussianConvolver.h:178-242  (H, V) lobe pairs of the six methods
 //   FIRGaussianConvolver.cu:52-98         detail::lobeSize (1 % threshold, size classes)
 //   TU/cu/algorithm.h:119-144             element conversion on tile load
@@ -372,6 +373,37 @@ void gauss_lobes(A sigma, A* lobe0, int* n0, A* lobe1, int* n1, A* lobe2, int* n
     *n0 = int(s0); *n1 = int(s1); *n2 = int(s2);
 }
 
+// Lobes for a radius fixed at r = ceil(4 sigma), clamped to [1, 16] (the
+// batched and served workloads ask for it; the reference has no such rule).
+// Symmetric lobes 0 and 2 get r + 1 entries, the antisymmetric lobe 1 gets r,
+// and each is normalised by the formula FIRGaussianConvolver.h:141-163
+// applies to the lengths lobeSize picks (lobe 1 keeps its off-by-one: taps
+// at offsets r .. 1, normalised over offsets 0 .. r-1).
+template <class A>
+void gauss_lobes_ceil4(A sigma, A* lobe0, int* n0, A* lobe1, int* n1, A* lobe2, int* n2)
+{
+    A lb0[LobeSizeMax], lb1[LobeSizeMax], lb2[LobeSizeMax];
+    for (size_t i = 0; i < LobeSizeMax; ++i) {
+        const A dx = A(i) / sigma, dxdx = dx * dx;
+        lb0[i] = std::exp(A(-0.5) * dxdx);
+        lb1[i] = -dx * lb0[i];
+        lb2[i] = (dxdx - A(1)) * lb0[i];
+    }
+    int r = int(std::ceil(A(4) * sigma));
+    r = r < 1 ? 1 : r > 16 ? 16 : r;
+    const size_t s0 = size_t(r) + 1, s1 = size_t(r), s2 = size_t(r) + 1;
+    A sum = lb0[0];
+    for (size_t i = 1; i < s0; ++i) sum += (A(2) * lb0[i]);
+    for (size_t i = 0; i < s0; ++i) lobe0[i] = lb0[s0 - 1 - i] / std::abs(sum);
+    sum = A(0);
+    for (size_t i = 0; i < s1; ++i) sum += (A(2) * A(i) * lb1[i]);
+    for (size_t i = 0; i < s1; ++i) lobe1[i] = lb1[s1 - i] / std::abs(sum);
+    sum = A(0);
+    for (size_t i = 1; i < s2; ++i) sum += (A(i * i) * lb2[i]);
+    for (size_t i = 0; i < s2; ++i) lobe2[i] = lb2[s2 - 1 - i] / std::abs(sum);
+    *n0 = int(s0); *n1 = int(s1); *n2 = int(s2);
+}
+
 }   // namespace
 
 extern "C" {
@@ -394,6 +426,16 @@ void orc_gauss_lobes_f32(float sigma, float* l0, int* n0, float* l1, int* n1, fl
 void orc_gauss_lobes_f64(double sigma, double* l0, int* n0, double* l1, int* n1, double* l2, int* n2)
 {
     gauss_lobes<double>(sigma, l0, n0, l1, n1, l2, n2);
+}
+
+void orc_gauss_lobes_ceil4_f32(float sigma, float* l0, int* n0, float* l1, int* n1, float* l2, int* n2)
+{
+    gauss_lobes_ceil4<float>(sigma, l0, n0, l1, n1, l2, n2);
+}
+
+void orc_gauss_lobes_ceil4_f64(double sigma, double* l0, int* n0, double* l1, int* n1, double* l2, int* n2)
+{
+    gauss_lobes_ceil4<double>(sigma, l0, n0, l1, n1, l2, n2);
 }
 
 int orc_fir2_f32(const float* lobeH, int nH, const float* lobeV, int nV,

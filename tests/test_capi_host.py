@@ -71,10 +71,18 @@ def test_lobe_size_equals_oracle():
             assert tucuda_amd.lobe_size(lobe, even) == oracle.lobe_size(lobe, even)
 
 
-def test_ceil4sigma_rule():
+def test_ceil4sigma_rule_equals_oracle_bitwise():
     """Extension rule: radius ceil(4 sigma) capped at 16, the reference's
-    normalisations (unit DC gain for lobe 0, quadratic response 2)."""
+    normalisations (unit DC gain for lobe 0, quadratic response 2); the host
+    library's float lobes equal the oracle's FP32 ceil(4 sigma) lobes bit for
+    bit (the oracle's rule is pinned in tests/test_oracle_lobes.py)."""
     import tucuda_amd
+    for sigma in np.linspace(0.5, 5.0, 181):
+        s = float(np.float32(sigma))
+        ours = tucuda_amd.gauss_lobes(s, tucuda_amd.TUCU_RULE_CEIL4SIGMA)
+        ref = oracle.gauss_lobes(s, "f32", "ceil4")
+        for a, b in zip(ours, ref):
+            assert a.size == b.size and (a.view(np.uint32) == b.view(np.uint32)).all(), sigma
     for sigma in (0.8, 1.0, 1.3, 2.2, 3.0, 3.99, 4.0):
         l0, l1, l2 = tucuda_amd.gauss_lobes(sigma, tucuda_amd.TUCU_RULE_CEIL4SIGMA)
         r = min(16, int(np.ceil(4 * np.float32(sigma))))

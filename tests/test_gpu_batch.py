@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._util import assert_f32_parity, full_taps, to_host
+from tests._util import assert_f32_parity, to_host
 from tucuda_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -43,10 +43,6 @@ def _single(dev, job):
     return ref
 
 
-def _full_taps(lobe, symmetric):
-    return full_taps(lobe, symmetric).astype(np.float32)
-
-
 @pytest.mark.parametrize("rule", [0, 1])
 def test_batch_matches_single_and_oracle(dev, rule):
     import torch
@@ -74,7 +70,7 @@ def test_batch_matches_single_and_oracle(dev, rule):
             if rule == 0:
                 oracle.fir2(lobes[h], lobes[v], img, want, shift)
             else:
-                oracle.fir2_taps(_full_taps(lobes[h], h != 1), _full_taps(lobes[v], v != 1), img, want, shift)
+                oracle.gauss_general(m, oracle.gauss_lobes(sigma, "f32", "ceil4"), img, want, shift)
             assert_f32_parity(to_host(t), want, f"sigma={sigma} {m} shift={shift}")
     b.close()
 

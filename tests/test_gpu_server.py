@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._util import bf16_bits_to_f32, full_taps, to_host
+from tests._util import bf16_bits_to_f32, to_host
 from tucuda_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -169,12 +169,10 @@ def test_served_bf16_against_oracle(dev):
         for rid, (x, sigma, _, _) in zip(ids, reqs):
             outs = _outputs(srv, got[rid], x, 0x3F)
             xin = bf16_bits_to_f32(to_host(x))
-            lobes = tucuda_amd.gauss_lobes(sigma, tucuda_amd.TUCU_RULE_CEIL4SIGMA)
+            lobes64 = oracle.gauss_lobes(sigma, "f64", "ceil4")      # FP64 coefficients too
             for k, m in enumerate(tucuda_amd.METHOD_ORDER):
-                h, v = oracle.METHODS[m]
                 want = np.full(xin.shape, np.nan, np.float32)
-                oracle.fir2_taps(full_taps(lobes[h], h != 1), full_taps(lobes[v], v != 1),
-                                 xin, want, True, precision="f64")
+                oracle.gauss_general(m, lobes64, xin, want, True, precision="f64")
                 valid = ~np.isnan(want)
                 got_k = bf16_bits_to_f32(to_host(outs[k])).astype(np.float64)
                 d = got_k[valid] - want[valid]

@@ -234,3 +234,77 @@ def test_general_taps_brute_force(nH, nV):
     oracle.fir2(lobe, lobe, img, a)
     oracle.fir2_taps(sym, sym, img, b)
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-4)
+
+
+def _closed_form_failures(method_out, sigma):
+    """Closed-form responses of the six methods to polynomial images, for the
+    (H, V) lobe pairs of FIRGaussianConvolver.h:181-241.  ``method_out(m,
+    img)`` returns method m's output (NaN outside the written region).  With
+    k_l the full kernel of lobe l: gain1 = sum_i k1[i] i (ramp response),
+    quad2 = sum_i k2[i] (i - c)^2, dc2 = sum_i k2[i], m2 = sum_i k0[i] (i - c)^2.
+    Returns the list of checks that fail."""
+    l0, l1, l2 = oracle.gauss_lobes(sigma, "f64")
+    k0, k1, k2 = (oracle.full_kernel(l) for l in (l0, l1, l2))
+    gain1 = float(np.dot(k1, np.arange(k1.size)))
+    c0, c2 = (k0.size - 1) // 2, (k2.size - 1) // 2
+    quad2 = float(np.dot(k2, (np.arange(k2.size) - c2) ** 2.0))
+    dc2 = float(k2.sum())
+    m2 = float(np.dot(k0, (np.arange(k0.size) - c0) ** 2.0))
+    nrow, ncol = 72, 80
+    yc = np.arange(nrow, dtype=np.float64)[:, None] - nrow / 2 + np.zeros((1, ncol))
+    xc = np.arange(ncol, dtype=np.float64)[None, :] - ncol / 2 + np.zeros((nrow, 1))
+    cases = [
+        # image, method, expected output at the same (centre) pixel
+        ("y^2 diffVV", yc ** 2, "diffVV", quad2 + dc2 * yc ** 2),
+        ("y^2 diffHH", yc ** 2, "diffHH", dc2 * (yc ** 2 + m2)),
+        ("x^2 diffHH", xc ** 2, "diffHH", quad2 + dc2 * xc ** 2),
+        ("x^2 diffVV", xc ** 2, "diffVV", dc2 * (xc ** 2 + m2)),
+        ("xy diffHV", xc * yc, "diffHV", np.full((nrow, ncol), gain1 * gain1)),
+        ("xy diffH", xc * yc, "diffH", gain1 * yc),
+        ("xy diffV", xc * yc, "diffV", gain1 * xc),
+        ("xy smooth", xc * yc, "smooth", xc * yc),
+    ]
+    bad = []
+    for name, img, m, want in cases:
+        got = method_out(m, img.astype(np.float32)).astype(np.float64)
+        valid = ~np.isnan(got)
+        assert valid.sum() > 100, name
+        scale = max(1.0, np.abs(want[valid]).max())
+        if np.abs(got[valid] - want[valid]).max() > 1e-6 * scale:
+            bad.append(name)
+    return bad
+
+
+@pytest.mark.parametrize("sigma", [1.0, 1.5, 2.0, 2.5, 3.0])
+def test_second_and_mixed_derivative_closed_forms(sigma):
+    """diffVV on a y^2 image, diffHV on an x*y image (and their transposed
+    counterparts) pin the (H, V) lobe pairs of diffHV and diffVV; the
+    oracle meets every check."""
+    lobes = oracle.gauss_lobes(sigma, "f64")
+
+    def method_out(m, img):
+        out = np.full(img.shape, np.nan, np.float32)
+        return oracle.gauss(m, sigma, img, out, True, "f64", lobes=lobes)
+
+    assert _closed_form_failures(method_out, sigma) == []
+
+
+@pytest.mark.parametrize("planted,caught", [
+    ({"diffVV": (2, 0)}, "y^2 diffVV"),        # diffVV with its lobes swapped
+    ({"diffHV": (1, 0)}, "xy diffHV"),         # diffHV with lobe 0 vertically
+    ({"diffHV": (0, 1)}, "xy diffHV"),
+    ({"diffHH": (0, 2)}, "x^2 diffHH"),
+    ({"diffH": (0, 1)}, "xy diffH"),
+])
+def test_planted_lobe_pair_mistakes_fail(planted, caught):
+    """The same checks catch a method wired to the wrong (H, V) lobe pair."""
+    sigma = 2.0
+    lobes = oracle.gauss_lobes(sigma, "f64")
+    pairs = dict(oracle.METHODS, **planted)
+
+    def method_out(m, img):
+        h, v = pairs[m]
+        out = np.full(img.shape, np.nan, np.float32)
+        return oracle.fir2(lobes[h], lobes[v], img, out, True, "f64")
+
+    assert caught in _closed_form_failures(method_out, sigma)

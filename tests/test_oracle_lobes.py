@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from tests._util import full_taps
 
 SIGMAS = np.round(np.arange(0.55, 4.6, 0.0125), 6)
 
@@ -121,3 +122,98 @@ def test_lobe_size_degenerate():
     e[0] = 1.0
     assert oracle.lobe_size(e, True) == 3
     assert oracle.lobe_size(e, False) == 2
+
+
+# ---------------------------------------------------------------------------
+# The ceil(4 sigma) radius rule (batched and served workloads): radius fixed
+# at r = ceil(4 sigma) clamped to [1, 16], the reference's normalisations.
+
+CEIL4_SIGMAS = [0.8, 0.9, 1.0, 1.13, 1.3, 1.6, 2.0, 2.2, 2.45, 2.8, 3.0, 3.3, 3.7, 3.99, 4.0]
+
+
+def _ceil4_radius(sigma: float, precision: str) -> int:
+    s = float(np.float32(sigma)) if precision == "f32" else sigma
+    return max(1, min(16, math.ceil(4 * s)))
+
+
+@pytest.mark.parametrize("sigma", CEIL4_SIGMAS)
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_ceil4_sizes_and_sample_shape(sigma, precision):
+    """Lengths r + 1, r, r + 1; every entry is the sampled Gaussian, its
+    first or its second derivative at the offset the entry stands for
+    (lobe[0] outermost), up to one positive factor per lobe."""
+    l0, l1, l2 = oracle.gauss_lobes(sigma, precision, "ceil4")
+    r = _ceil4_radius(sigma, precision)
+    assert (l0.size, l1.size, l2.size) == (r + 1, r, r + 1)
+    s = float(np.float32(sigma)) if precision == "f32" else sigma
+    tol = 5e-6 if precision == "f32" else 1e-12
+    off0 = np.arange(r, -1, -1, dtype=np.float64)          # offsets of lobe0 / lobe2 entries
+    off1 = np.arange(r, 0, -1, dtype=np.float64)           # offsets of lobe1 entries
+    g = lambda o: np.exp(-0.5 * (o / s) ** 2)
+    want0 = g(off0)
+    want1 = -(off1 / s) * g(off1)
+    want2 = ((off0 / s) ** 2 - 1) * g(off0)
+    for got, want in ((l0, want0), (l1, want1), (l2, want2)):
+        got = got.astype(np.float64)
+        k = np.argmax(np.abs(want))
+        factor = got[k] / want[k]
+        assert factor > 0
+        np.testing.assert_allclose(got, factor * want, rtol=tol, atol=tol * np.abs(got).max())
+
+
+@pytest.mark.parametrize("sigma", CEIL4_SIGMAS)
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_ceil4_normalisation_identities(sigma, precision):
+    """The identities the reference's normalisation builds in
+    (FIRGaussianConvolver.h:141-163), at the fixed radius: unit DC gain of
+    lobe 0; lobe 1's ramp response 1 + 2 r |g1(r)| / sum_{i<r} 2 i |g1(i)|
+    (taps at offsets r .. 1, normalised over 0 .. r-1) and no DC response;
+    lobe 2's quadratic response +-2 and its DC in closed form."""
+    l0, l1, l2 = oracle.gauss_lobes(sigma, precision, "ceil4")
+    r = l1.size
+    s = float(np.float32(sigma)) if precision == "f32" else sigma
+    tol = 3e-6 if precision == "f32" else 1e-13
+    # lobe roles, not lengths, fix the parity here (r + 1 is even for odd r)
+    k0, k1, k2 = full_taps(l0, True), full_taps(l1, False), full_taps(l2, True)
+    i = np.arange(17, dtype=np.float64)
+    g0 = np.exp(-0.5 * (i / s) ** 2)
+    g1 = -(i / s) * g0
+    g2 = ((i / s) ** 2 - 1) * g0
+    assert abs(k0.sum() - 1.0) < tol
+    ramp = float(np.dot(k1, np.arange(k1.size, dtype=np.float64)))
+    want = 1.0 + 2 * r * abs(g1[r]) / sum(2 * j * abs(g1[j]) for j in range(r))
+    assert abs(ramp - want) < 20 * tol, (ramp, want)
+    assert abs(k1.sum()) < tol
+    den = sum(j * j * g2[j] for j in range(1, r + 1))
+    quad = float(np.dot(k2, (np.arange(k2.size) - r) ** 2.0))
+    assert abs(quad - 2.0 * np.sign(den)) < 40 * tol, quad
+    dc = (g2[0] + 2 * g2[1:r + 1].sum()) / abs(den)
+    assert abs(k2.sum() - dc) < 40 * tol
+
+
+def test_ceil4_f32_and_f64_agree():
+    for sigma in CEIL4_SIGMAS:
+        for a, b in zip(oracle.gauss_lobes(sigma, "f32", "ceil4"), oracle.gauss_lobes(sigma, "f64", "ceil4")):
+            assert a.size == b.size
+            np.testing.assert_allclose(a, b, rtol=3e-6, atol=1e-8)
+
+
+def test_ceil4_clamps():
+    """Radii below 1 and above 16 are clamped."""
+    assert [x.size for x in oracle.gauss_lobes(0.1, "f64", "ceil4")] == [2, 1, 2]
+    assert [x.size for x in oracle.gauss_lobes(9.0, "f64", "ceil4")] == [17, 16, 17]
+
+
+def test_planted_ceil4_mistakes_fail():
+    """A lobe 1 normalised over offsets 1 .. r (no off-by-one) or a radius
+    one short breaks the checks above: the identities pin the rule."""
+    sigma, r = 2.2, 9
+    s = sigma
+    i = np.arange(17, dtype=np.float64)
+    g1 = -(i / s) * np.exp(-0.5 * (i / s) ** 2)
+    mutant = g1[r:0:-1] / abs(sum(2 * j * g1[j] for j in range(1, r + 1)))     # normalised over 1..r
+    k1 = full_taps(mutant, False)
+    ramp = float(np.dot(k1, np.arange(k1.size, dtype=np.float64)))
+    want = 1.0 + 2 * r * abs(g1[r]) / sum(2 * j * abs(g1[j]) for j in range(r))
+    assert abs(ramp - want) > 1e-3
+    assert oracle.gauss_lobes(sigma, "f64", "ceil4")[1].size == r != r - 1
