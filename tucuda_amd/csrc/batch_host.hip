@@ -23,9 +23,14 @@ BatchEntry g_batch_mfma_table[NBTH][4][4];
 
 namespace host {
 
-const BatchEntry& batch_entry(int thi, int in_dt, int out_dt)
+int batch_family()
 {
-    return gauss_use_mfma() ? g_batch_mfma_table[thi][in_dt][out_dt] : g_batch_table[thi][in_dt][out_dt];
+    return gauss_use_mfma() ? BATCH_MFMA : BATCH_VALU;
+}
+
+const BatchEntry& batch_entry(int fam, int thi, int in_dt, int out_dt)
+{
+    return fam == BATCH_MFMA ? g_batch_mfma_table[thi][in_dt][out_dt] : g_batch_table[thi][in_dt][out_dt];
 }
 
 // Tile list of `jobs` for the tile shape of e (sets every job's tiles_y):
@@ -76,9 +81,10 @@ struct Layout {
     bool       built   = false;
 };
 
-// Tuning cache: (in dtype, out dtype, radius class of the largest job,
-// log2 of the tile count at the default height) -> best tile-height index.
-using TuneKey = std::tuple<int, int, int, int>;
+// Tuning cache: (kernel family, in dtype, out dtype, radius class of the
+// largest job, log2 of the tile count at the default height) -> best
+// tile-height index.
+using TuneKey = std::tuple<int, int, int, int, int>;
 struct TuneVal { int th_index; float us; };
 std::mutex                  g_tune_mutex;
 std::map<TuneKey, TuneVal>  g_tune;
@@ -90,6 +96,7 @@ using namespace tucu;
 using namespace tucu::host;
 
 struct tucu_batch_s {
+    int                  fam = 0;                // kernel family, fixed at create
     int                  in_dt = 0, out_dt = 0;
     std::vector<SepArgs> jobs;
     Layout               layout[NBTH];
@@ -114,7 +121,7 @@ int build_layout(tucu_batch_s& b, int thi)
 {
     Layout& l = b.layout[thi];
     if (l.built) return TUCU_OK;
-    const BatchEntry& e = batch_entry(thi, b.in_dt, b.out_dt);
+    const BatchEntry& e = batch_entry(b.fam, thi, b.in_dt, b.out_dt);
     if (!e.fn) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
     std::vector<SepArgs> jobs = b.jobs;
     std::vector<BatchTile> all;
@@ -143,7 +150,7 @@ int pick_th(const tucu_batch_s& b)
 {
     if (b.th_index >= 0) return b.th_index;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    const auto it = g_tune.find(TuneKey(b.in_dt, b.out_dt, b.rclass, b.bucket));
+    const auto it = g_tune.find(TuneKey(b.fam, b.in_dt, b.out_dt, b.rclass, b.bucket));
     return it != g_tune.end() ? it->second.th_index : BATCH_TH_DEFAULT;
 }
 
@@ -153,7 +160,7 @@ int launch_layout(tucu_batch_s& b, int thi, hipStream_t s)
     if ((rc = build_layout(b, thi))) return rc;
     const Layout& l = b.layout[thi];
     if (l.ntiles == 0) return TUCU_OK;
-    const hipError_t e = batch_entry(thi, b.in_dt, b.out_dt).fn(l.d_jobs, l.d_tiles, l.ntiles,
+    const hipError_t e = batch_entry(b.fam, thi, b.in_dt, b.out_dt).fn(l.d_jobs, l.d_tiles, l.ntiles,
                                                                  b.d_counter, l.grid, s);
     return e == hipSuccess ? TUCU_OK : device_fail(e, "batched kernel launch");
 }
@@ -168,6 +175,7 @@ int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
     *batch = nullptr;
     if (njobs < 0 || (njobs > 0 && !jobs)) return fail(TUCU_ERR_INVALID, "bad job array");
     auto b = new tucu_batch_s();
+    b->fam = batch_family();
     int rmax_all = 0;
     int64_t work = 0;
     for (int j = 0; j < njobs; ++j) {
@@ -179,6 +187,12 @@ int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
         const int rc = gauss_args(lobes, n, &g.in, g.out, g.mask, g.shift, a, rmax, odt);
         if (rc) { delete b; return rc; }
         if (!a.mask) continue;                                  // nothing to write for this job
+        if (const int al = gauss_aliasing(&g.in, g.out, a.mask, a.r, g.shift != 0)) {
+            delete b;
+            return al == -1 ? fail(TUCU_ERR_INVALID, "job " + std::to_string(j) +
+                                                     ": an output overlaps the input (batched jobs cannot filter in place)")
+                            : al;
+        }
         if (b->jobs.empty()) { b->in_dt = g.in.dtype; b->out_dt = odt; }
         else if (g.in.dtype != b->in_dt || odt != b->out_dt) {
             delete b;
@@ -222,8 +236,8 @@ int tucu_batch_set_tile_rows(tucu_batch batch, int rows)
         thi = pick_th(*batch);
     } else {
         for (int i = 0; i < NBTH; ++i)
-            if (batch_entry(i, batch->in_dt, batch->out_dt).th == rows &&
-                batch_entry(i, batch->in_dt, batch->out_dt).fn) thi = i;
+            if (batch_entry(batch->fam, i, batch->in_dt, batch->out_dt).th == rows &&
+                batch_entry(batch->fam, i, batch->in_dt, batch->out_dt).fn) thi = i;
         batch->th_index = thi;
     }
     if (thi >= 0) {
@@ -250,7 +264,7 @@ int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* t
     if (njobs) *njobs = int(batch->jobs.size());
     if (ntiles) *ntiles = l.ntiles;
     if (grid) *grid = l.grid;
-    if (tile_rows) *tile_rows = batch_entry(thi, batch->in_dt, batch->out_dt).th;
+    if (tile_rows) *tile_rows = batch_entry(batch->fam, thi, batch->in_dt, batch->out_dt).th;
     return TUCU_OK;
 }
 
@@ -270,7 +284,7 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
     float best_t = 0.f;
     int rc = TUCU_OK;
     for (int i = 0; i < NBTH && rc == TUCU_OK; ++i) {
-        if (!batch_entry(i, batch->in_dt, batch->out_dt).fn) continue;
+        if (!batch_entry(batch->fam, i, batch->in_dt, batch->out_dt).fn) continue;
         if ((rc = launch_layout(*batch, i, s))) break;                  // builds and warms up
         std::vector<float> t(reps);
         for (int r = 0; r < reps && rc == TUCU_OK; ++r) {
@@ -293,9 +307,9 @@ int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows
     batch->active = best;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
-        g_tune[TuneKey(batch->in_dt, batch->out_dt, batch->rclass, batch->bucket)] = {best, best_t};
+        g_tune[TuneKey(batch->fam, batch->in_dt, batch->out_dt, batch->rclass, batch->bucket)] = {best, best_t};
     }
-    if (best_rows) *best_rows = batch_entry(best, batch->in_dt, batch->out_dt).th;
+    if (best_rows) *best_rows = batch_entry(batch->fam, best, batch->in_dt, batch->out_dt).th;
     if (best_us) *best_us = best_t;
     return TUCU_OK;
 }
@@ -306,11 +320,13 @@ int tucu_tune_cache_save(const char* path)
     std::ofstream f(path);
     if (!f) return fail(TUCU_ERR_INVALID, std::string("cannot write ") + path);
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    for (const auto& kv : g_tune)
-        f << "{\"in\": " << std::get<0>(kv.first) << ", \"out\": " << std::get<1>(kv.first)
-          << ", \"radius\": " << std::get<2>(kv.first) << ", \"log2_pixels\": " << std::get<3>(kv.first)
-          << ", \"tile_rows\": " << batch_entry(kv.second.th_index, std::get<0>(kv.first), std::get<1>(kv.first)).th
+    for (const auto& kv : g_tune) {
+        const int fam = std::get<0>(kv.first), in = std::get<1>(kv.first), out = std::get<2>(kv.first);
+        f << "{\"family\": " << fam << ", \"in\": " << in << ", \"out\": " << out
+          << ", \"radius\": " << std::get<3>(kv.first) << ", \"log2_pixels\": " << std::get<4>(kv.first)
+          << ", \"tile_rows\": " << batch_entry(fam, kv.second.th_index, in, out).th
           << ", \"us\": " << kv.second.us << "}\n";
+    }
     return TUCU_OK;
 }
 
@@ -322,14 +338,16 @@ int tucu_tune_cache_load(const char* path)
     std::string line;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     while (std::getline(f, line)) {
-        int in = -1, out = -1, r = -1, bucket = -1, rows = -1;
+        int fam = -1, in = -1, out = -1, r = -1, bucket = -1, rows = -1;
         float us = 0.f;
-        if (std::sscanf(line.c_str(), "{\"in\": %d, \"out\": %d, \"radius\": %d, \"log2_pixels\": %d, "
-                                      "\"tile_rows\": %d, \"us\": %f}", &in, &out, &r, &bucket, &rows, &us) != 6)
+        if (std::sscanf(line.c_str(), "{\"family\": %d, \"in\": %d, \"out\": %d, \"radius\": %d, "
+                                      "\"log2_pixels\": %d, \"tile_rows\": %d, \"us\": %f}",
+                        &fam, &in, &out, &r, &bucket, &rows, &us) != 7)
             continue;
+        if (fam < 0 || fam > 1 || in < 0 || in > 3 || out < 0 || out > 3) continue;
         for (int i = 0; i < NBTH; ++i)
-            if (in >= 0 && in < 4 && out >= 0 && out < 4 && batch_entry(i, in, out).fn && batch_entry(i, in, out).th == rows)
-                g_tune[TuneKey(in, out, r, bucket)] = {i, us};
+            if (batch_entry(fam, i, in, out).fn && batch_entry(fam, i, in, out).th == rows)
+                g_tune[TuneKey(fam, in, out, r, bucket)] = {i, us};
     }
     return TUCU_OK;
 }

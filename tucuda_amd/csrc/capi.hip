@@ -259,6 +259,83 @@ int check_extent(const tucu_image& o, int rv, int rh, int nrow, int ncol, bool s
     return TUCU_OK;
 }
 
+// ---------------------------------------------------------------- aliasing
+// The reference's FIRFilter2::convolve runs its horizontal pass into _buf
+// and its vertical pass from _buf into the output (FIRFilter.h:298-377), so
+// a call whose output is its own input is well defined there.  The fused
+// launches here read the input's halo while other workgroups write outputs,
+// so an output that overlaps the input goes through a copy of the input.
+
+// Bytes [lo, hi) an image spans, rows r0 .. r0 + nr - 1, columns c0 .. c0 + nc - 1.
+struct Span { uintptr_t lo = 0, hi = 0; };
+
+Span span_of(const tucu_image& im, int64_t r0, int64_t nr, int64_t c0, int64_t nc)
+{
+    Span s;
+    if (nr <= 0 || nc <= 0 || !im.data) return s;
+    const int64_t es = int64_t(dtype_size(im.dtype));
+    s.lo = reinterpret_cast<uintptr_t>(im.data) + uintptr_t((r0 * im.stride + c0) * es);
+    s.hi = reinterpret_cast<uintptr_t>(im.data) + uintptr_t(((r0 + nr - 1) * im.stride + c0 + nc) * es);
+    return s;
+}
+
+bool overlap(const Span& a, const Span& b) { return a.lo < b.hi && b.lo < a.hi; }
+
+Span input_span(const tucu_image& in) { return span_of(in, 0, in.nrow, 0, in.ncol); }
+
+// Region a FIRFilter2-style call writes: (nrow - 2 rv) x (ncol - 2 rh) at
+// (rv, rh) when shifted, at (0, 0) otherwise (FIRFilter.h:286-287).
+Span written_span(const tucu_image& o, int rv, int rh, int nrow, int ncol, bool shift)
+{
+    return span_of(o, shift ? rv : 0, nrow - 2 * rv, shift ? rh : 0, ncol - 2 * rh);
+}
+
+// A device copy of an input image, made on the call's stream when an output
+// overlaps it, freed once the call's work is done (the call then waits for
+// its stream: in-place calls are synchronous).
+struct InputCopy {
+    void*      tmp = nullptr;
+    tucu_image img{};
+
+    int make(const tucu_image& in, hipStream_t s)
+    {
+        const size_t es = dtype_size(in.dtype);
+        const size_t stride = tucu_row_stride(size_t(in.ncol), es);
+        if (hipMalloc(&tmp, std::max<size_t>(1, size_t(in.nrow) * stride * es)) != hipSuccess) {
+            (void)hipGetLastError();
+            tmp = nullptr;
+            return fail(TUCU_ERR_BAD_ALLOC, "device allocation for the copy of an aliased input failed");
+        }
+        const hipError_t e = hipMemcpy2DAsync(tmp, stride * es, in.data, size_t(in.stride) * es,
+                                              size_t(in.ncol) * es, size_t(in.nrow), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return device_fail(e, "hipMemcpy2DAsync (aliased input)");
+        img = in;
+        img.data = tmp;
+        img.stride = int64_t(stride);
+        return TUCU_OK;
+    }
+
+    // Waits for the stream's work, then frees the copy; passes rc through
+    // unless waiting fails.
+    int finish(int rc, hipStream_t s)
+    {
+        if (!tmp) return rc;
+        const hipError_t e = hipStreamSynchronize(s);
+        (void)hipFree(tmp);
+        tmp = nullptr;
+        if (rc == TUCU_OK && e != hipSuccess) return device_fail(e, "hipStreamSynchronize (aliased input)");
+        return rc;
+    }
+
+    ~InputCopy()
+    {
+        if (tmp) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(tmp);
+        }
+    }
+};
+
 int launch(const KernelEntry& k, SepArgs& a, hipStream_t s)
 {
     if (!k.fn) return fail(TUCU_ERR_INVALID, "no kernel for this type / radius combination");
@@ -340,6 +417,28 @@ int gauss_args(const float* const lobes[3], const int n[3], const tucu_image* in
     mfma_scales(a);
     a.mask = live;
     a.rt = GAUSS_RADII[gauss_class(rmax)];
+    return TUCU_OK;
+}
+
+// Aliasing of a fused Gaussian call whose live outputs are `live` (radii
+// r[] of lobes 0, 1, 2): TUCU_ERR_INVALID if two outputs overlap (one
+// launch writes both, so neither result would hold), -1 if an output
+// overlaps the input, TUCU_OK otherwise.
+int gauss_aliasing(const tucu_image* in, const tucu_image* outs, unsigned live, const int* r, bool shift)
+{
+    static const int HV[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};
+    Span w[6];
+    for (int k = 0; k < 6; ++k)
+        if (live & (1u << k))
+            w[k] = written_span(outs[k], r[HV[k][1]], r[HV[k][0]], in->nrow, in->ncol, shift);
+    for (int k = 0; k < 6; ++k)
+        for (int j = 0; j < k; ++j)
+            if ((live >> k & 1u) && (live >> j & 1u) && overlap(w[k], w[j]))
+                return fail(TUCU_ERR_INVALID, "outputs " + std::to_string(j) + " and " + std::to_string(k) +
+                                              " of one call overlap");
+    const Span src = input_span(*in);
+    for (int k = 0; k < 6; ++k)
+        if ((live >> k & 1u) && overlap(w[k], src)) return -1;
     return TUCU_OK;
 }
 
@@ -434,6 +533,12 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
     if (!lobeH || !lobeV) return fail(TUCU_ERR_INVALID, "null lobe");
     if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
     if ((rc = check_extent(*out, rV, rH, nrow, ncol, shift != 0, "output"))) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    InputCopy copy;
+    if (overlap(written_span(*out, rV, rH, nrow, ncol, shift != 0), input_span(*in))) {   // in place
+        if ((rc = copy.make(*in, st))) return copy.finish(rc, st);
+        in = &copy.img;
+    }
 
     SepArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -445,8 +550,8 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
     a.r[0] = rV; a.r[1] = rH;
     centre_index(lobeV, rV, nV & 1, a.cv[0]);
     centre_index(lobeH, rH, nH & 1, a.ch);
-    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
-    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
+    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return copy.finish(rc, st);
+    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return copy.finish(rc, st);
     if (fir_use_mfma()) {
         // full taps: c[k] on the upper / left side, sgn * c[k] on the other (FIRFilter.h:66-176)
         float vt[TMAX] = {0.f}, ht[TMAX] = {0.f};
@@ -454,10 +559,10 @@ int tucu_fir2_convolve(const float* lobeH, int nH, const float* lobeV, int nV,
             vt[16 - k] = a.cv[0][k]; vt[16 + k] = k ? a.sgnv * a.cv[0][k] : a.cv[0][0];
             ht[16 - k] = a.ch[k];    ht[16 + k] = k ? a.sgnh * a.ch[k] : a.ch[0];
         }
-        return launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, static_cast<hipStream_t>(stream));
+        return copy.finish(launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, st), st);
     }
     const KernelEntry& k = g_fir_table[fir_class(rV)][fir_class(rH)][in->dtype][out->dtype];
-    return launch(k, a, static_cast<hipStream_t>(stream));
+    return copy.finish(launch(k, a, st), st);
 }
 
 int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int nV,
@@ -474,6 +579,12 @@ int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int 
     if (!tapsH || !tapsV) return fail(TUCU_ERR_INVALID, "null taps");
     if (!in->data || !out->data) return fail(TUCU_ERR_INVALID, "null image data");
     if ((rc = check_extent(*out, rV, rH, nrow, ncol, shift != 0, "output"))) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    InputCopy copy;
+    if (overlap(written_span(*out, rV, rH, nrow, ncol, shift != 0), input_span(*in))) {   // in place
+        if ((rc = copy.make(*in, st))) return copy.finish(rc, st);
+        in = &copy.img;
+    }
     // template radius: the smallest class that holds the taps (a centred
     // 1-tap filter still uses the radius-2 kernel)
     auto cls = [](int r) { return r <= 2 ? 0 : r <= 4 ? 1 : r <= 8 ? 2 : 3; };
@@ -487,16 +598,16 @@ int tucu_fir2_convolve_taps(const float* tapsH, int nH, const float* tapsV, int 
     a.r[0] = rV; a.r[1] = rH;
     for (int i = 0; i < nV; ++i) a.tv[RV - rV + i] = tapsV[i];
     for (int i = 0; i < nH; ++i) a.th[RH - rH + i] = tapsH[i];
-    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return rc;
-    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return rc;
+    if ((rc = set_out(a, 0, *out, rV, rH, shift != 0))) return copy.finish(rc, st);
+    if ((rc = finish_args(a, *in, 1u, out, &rV, &rH))) return copy.finish(rc, st);
     if (fir_use_mfma()) {
         float vt[TMAX] = {0.f}, ht[TMAX] = {0.f};
         for (int i = 0; i < nV; ++i) vt[16 - rV + i] = tapsV[i];
         for (int i = 0; i < nH; ++i) ht[16 - rH + i] = tapsH[i];
-        return launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, static_cast<hipStream_t>(stream));
+        return copy.finish(launch_fir_mfma(a, vt, ht, rV, rH, in->dtype, out->dtype, st), st);
     }
     const KernelEntry& k = g_taps_table[cls(rV)][cls(rH)][in->dtype][out->dtype];
-    return launch(k, a, static_cast<hipStream_t>(stream));
+    return copy.finish(launch(k, a, st), st);
 }
 
 int tucu_lobe_size(const float* lobe17, int even)
@@ -532,15 +643,23 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
     int rmax = 0, odt = 0;
     int rc = gauss_args(lobes, n, in, outs, mask, shift, a, rmax, odt);
     if (rc || !a.mask) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    InputCopy copy;
+    if ((rc = gauss_aliasing(in, outs, a.mask, a.r, shift != 0))) {
+        if (rc != -1) return rc;
+        if ((rc = copy.make(*in, st))) return copy.finish(rc, st);     // an output overlaps the input
+        in = &copy.img;
+        if ((rc = gauss_args(lobes, n, in, outs, mask, shift, a, rmax, odt))) return copy.finish(rc, st);
+    }
     if (gauss_use_mfma()) {
         const int rc = rmax <= 8 ? 0 : 1;
         const KernelEntry* m = &g_gauss_mfma_table[rc][mfma_wy_index()][in->dtype][odt];
         if (!m->fn) m = &g_gauss_mfma_table[rc][MFMA_WY_DEFAULT][in->dtype][odt];
-        return launch(*m, a, static_cast<hipStream_t>(stream));
+        return copy.finish(launch(*m, a, st), st);
     }
     const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
     if (!e->fn) e = &g_gauss_table[gauss_class(rmax)][GAUSS_TH_DEFAULT][in->dtype][odt];
-    return launch(*e, a, static_cast<hipStream_t>(stream));
+    return copy.finish(launch(*e, a, st), st);
 }
 
 int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* stream)
@@ -558,8 +677,14 @@ int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* st
         return fail(TUCU_ERR_INVALID, "output: smaller than nrow/2 x ncol/2");
     if (out->stride > INT32_MAX) return fail(TUCU_ERR_INVALID, "output row stride too large");
     const int64_t es = int64_t(dtype_size(in->dtype));
+    if (int64_t(nrow) * in->stride * es > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    InputCopy copy;
+    if (overlap(span_of(*out, 0, nrow / 2, 0, ncol / 2), input_span(*in))) {       // in place
+        if ((rc = copy.make(*in, st))) return copy.finish(rc, st);
+        in = &copy.img;
+    }
     const int64_t stride_b = in->stride * es;
-    if (int64_t(nrow) * stride_b > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
 
     SubArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -573,9 +698,9 @@ int tucu_subsample(const tucu_image* in, const tucu_image* out, int op, void* st
     a.ostride = int32_t(out->stride);
     a.out = reinterpret_cast<uintptr_t>(out->data);
     const SubLaunchFn fn = g_sub_table[op][in->dtype][out->dtype];
-    if (!fn) return fail(TUCU_ERR_INVALID, "no subsample kernel for this operator / type combination");
-    const hipError_t e = fn(a, static_cast<hipStream_t>(stream));
-    return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
+    if (!fn) return copy.finish(fail(TUCU_ERR_INVALID, "no subsample kernel for this operator / type combination"), st);
+    const hipError_t e = fn(a, st);
+    return copy.finish(e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch"), st);
 }
 
 int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in, const tucu_image* out,
@@ -594,8 +719,14 @@ int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in, const 
         return fail(TUCU_ERR_INVALID, "output: smaller than nrow/2 x ncol/2");
     if (out->stride > INT32_MAX) return fail(TUCU_ERR_INVALID, "output row stride too large");
     const int64_t es = int64_t(dtype_size(in->dtype));
+    if (int64_t(nrow) * in->stride * es > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    InputCopy copy;
+    if (overlap(span_of(*out, 0, nrow / 2, 0, ncol / 2), input_span(*in))) {       // in place
+        if ((rc = copy.make(*in, st))) return copy.finish(rc, st);
+        in = &copy.img;
+    }
     const int64_t stride_b = in->stride * es;
-    if (int64_t(nrow) * stride_b > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
 
     PyrArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -612,9 +743,9 @@ int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in, const 
     centre_index(lobe0, r, true, a.c);
     const int ri = r <= 4 ? 0 : r <= 8 ? 1 : 2;
     const PyrLaunchFn fn = g_pyr_table[ri][in->dtype][out->dtype];
-    if (!fn) return fail(TUCU_ERR_INVALID, "no kernel for this type combination");
-    const hipError_t e = fn(a, static_cast<hipStream_t>(stream));
-    return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
+    if (!fn) return copy.finish(fail(TUCU_ERR_INVALID, "no kernel for this type combination"), st);
+    const hipError_t e = fn(a, st);
+    return copy.finish(e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch"), st);
 }
 
 }   // extern "C"

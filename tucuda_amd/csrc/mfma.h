@@ -35,8 +35,10 @@
 // two terms (a peak of 2^12 left the inputs at a quarter and their lo
 // halves subnormal below half the strip's largest value: 20x the error on
 // an image whose strips mix 0..255 data with values of 10^6).
-// Non-finite inputs make every output of their 16 x 16 block window
-// non-finite (the band products multiply them by zero).
+// A strip whose input holds a NaN or an Inf, or whose largest magnitude the
+// scale cannot bring into range, is filtered by the FP32 form
+// (mfma_strip_fp32), which reaches exactly the outputs whose window holds
+// a non-finite input, as the reference's tap sums do.
 #pragma once
 
 #include "sepfir.h"
@@ -234,6 +236,134 @@ __device__ __forceinline__ v8f16 pairP(const V& f)
     return pair8<2 * P>(f);
 }
 
+// ------------------------------------------------------------------------
+// FP32 form of one strip, for input the split-f16 products cannot carry:
+// non-finite values (a band product multiplies every input of a block by
+// every band entry, zeros included, so one NaN or Inf would reach all 16 x
+// 16 outputs of its block window, and Inf - Inf in the hi / lo split turns
+// Inf into NaN), or a strip whose largest magnitude lies outside the range
+// its power-of-two scale covers.  Each output is summed over exactly the
+// taps of its lobes -- the true radius, zero taps inside it included, none
+// beyond it, no centre tap for an antisymmetric lobe, paired as
+// FIRFilter.h:66-176 pairs them -- so a NaN or an Inf reaches exactly the
+// outputs whose window holds it, with the sign the reference's sums give.
+// Vertical pass first, one half strip (32 output columns, 32 + 2 RC field
+// columns) at a time through the wave's staging area.  Kept out of line so
+// that the matrix-core path's register allocation and schedule are its own.
+
+// Tap sum over w(-R) .. w(R) around w(0) of a lobe of radius r <= R whose
+// tap at distance k is c(k) on the upper / left side: symmetric: sum_k c(k)
+// (w(-k) + w(k)) + c(0) w(0); antisymmetric: sum_k c(k) (w(-k) - w(k));
+// outermost tap first.  Taps beyond r are skipped, never multiplied.
+template <int R, class C, class W>
+__device__ __forceinline__ float fp32_paired(const C& c, int r, bool sym, const W& w)
+{
+    float acc = 0.f;
+#pragma unroll
+    for (int k = R; k >= 1; --k) {
+        const float lo = w(-k), hi = w(k);
+        const float s = sym ? lo + hi : lo - hi;
+        acc = k <= r ? (k == r ? c(k) * s : __builtin_fmaf(c(k), s, acc)) : acc;
+    }
+    return sym ? __builtin_fmaf(c(0), w(0), acc) : acc;
+}
+
+// General taps t(o), o = -r .. r, top / left tap first.
+template <int R, class C, class W>
+__device__ __forceinline__ float fp32_taps(const C& t, int r, const W& w)
+{
+    float acc = 0.f;
+#pragma unroll
+    for (int o = -R; o <= R; ++o) {
+        const int k = o < 0 ? -o : o;
+        acc = k <= r ? (o == -r ? t(o) * w(o) : __builtin_fmaf(t(o), w(o), acc)) : acc;
+    }
+    return acc;
+}
+
+// The taps come from the lobe words staged in LDS (MfmaTabs::lobe): MODE_GAUSS
+// c_l(k) at 17 l + k; MODE_TAPS the full vertical taps at 16 + o and the
+// horizontal ones at 33 + 16 + o, which for FIRFilter2 lobes (sgnv, sgnh
+// nonzero) are c(k) on the upper / left side.
+template <int RC, class IN, class OUT, int MODE, class A>
+__device__ __forceinline__ void
+mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage)
+{
+    constexpr bool ONE = MODE == MODE_TAPS;
+    constexpr int NFC = 32 + 2 * RC;            // field columns per half strip
+    static_assert(NFC <= MOPITCH, "staging area too narrow");
+    const int lane = threadIdx.x & 63;
+    const int es = int(sizeof(IN));
+    const int sb = a.in_stride_b;
+    const unsigned mask = a.mask;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
+    constexpr int HL[6] = {ONE ? 1 : 0, 1, 0, 2, 1, 0};   // horizontal lobe of output k (TAPS: table 1)
+    constexpr int VL[6] = {0, 0, 1, 0, 1, 2};             // vertical lobe of output k
+    const unsigned fneed[3] = {ONE ? 1u : mask & 0x0bu, ONE ? 0u : mask & 0x14u, ONE ? 0u : mask & 0x20u};
+    const float* const lw = T.lobe;
+
+#pragma unroll 1
+    for (int l = 0; l < 3; ++l) {
+        if (!fneed[l]) continue;
+#pragma unroll 1
+        for (int q = 0; q < 2; ++q) {
+            // vertical pass: lane = field column x0 + 32 q - RC + lane, rows y0 .. y0 + 15;
+            // one row at a time, its window loaded again (L2-resident), so that
+            // this rarely taken path stays small in registers
+            const int col = x0 + 32 * q - RC + lane;
+            if (lane < NFC) {
+                const bool colok = unsigned(col) < unsigned(a.ncol);
+#pragma unroll 1
+                for (int i = 0; i < 16; ++i) {
+                    const auto win = [&](int d) {
+                        const int row = y0 + i + d;
+                        const bool ok = colok && unsigned(row) < unsigned(a.nrow);
+                        return Px<IN>::ld(rsrc, ok ? row * sb + col * es : MINVALID, 0);
+                    };
+                    float f;
+                    if constexpr (ONE) {
+                        f = a.sgnv != 0.f ? fp32_paired<RC>([&](int k) { return lw[16 - k]; }, a.r[0], a.sgnv > 0.f, win)
+                                          : fp32_taps<RC>([&](int o) { return lw[16 + o]; }, a.r[0], win);
+                    } else {
+                        f = fp32_paired<RC>([&](int k) { return lw[17 * l + k]; }, a.r[l], l != 1, win);
+                    }
+                    stage[i * MOPITCH + lane] = f;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // horizontal pass: lane = output column x0 + 32 q + (lane & 31), rows (lane >> 5) * 8 + 0 .. 7
+            const int j = lane & 31, i0 = (lane >> 5) * 8;
+            const int x = x0 + 32 * q + j;
+#pragma unroll 1
+            for (int k = 0; k < (ONE ? 1 : 6); ++k) {
+                if (VL[k] != l || !(mask >> k & 1u)) continue;
+                const int h = HL[k];
+                const int rv = ONE ? a.r[0] : a.r[VL[k]], rh = ONE ? a.r[1] : a.r[h];
+#pragma unroll 1
+                for (int i = 0; i < 8; ++i) {
+                    const float* srow = stage + (i0 + i) * MOPITCH + RC + j;
+                    const auto win = [&](int d) { return srow[d]; };
+                    float v;
+                    if constexpr (ONE) {
+                        v = a.sgnh != 0.f ? fp32_paired<RC>([&](int kk) { return lw[49 - kk]; }, a.r[1], a.sgnh > 0.f, win)
+                                          : fp32_taps<RC>([&](int o) { return lw[49 + o]; }, a.r[1], win);
+                    } else {
+                        v = fp32_paired<RC>([&](int kk) { return lw[17 * h + kk]; }, a.r[h], h != 1, win);
+                    }
+                    const int y = y0 + i0 + i;
+                    if (y >= rv && y < a.nrow - rv && x >= rh && x < a.ncol - rh) {
+                        const gptr<OUT> p = reinterpret_cast<gptr<OUT>>(
+                            a.obase[k] + uintptr_t(int64_t(y) * a.ostride[k] + x) * sizeof(OUT));
+                        Px<OUT>::st(p, v);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 #define TUCU_MFMA(A_, B_, C_) __builtin_amdgcn_mfma_f32_16x16x32_f16((A_), (B_), (C_), 0, 0, 0)
 
 // One strip: centre rows y0 .. y0+15, centre columns x0 .. x0+63.  T holds
@@ -333,7 +463,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     int ex;
     (void)frexpf(mx, &ex);                      // mx < 2^ex
     ex = 13 - ex;                               // largest scaled input in [2^12, 2^13)
-    ex = ex < -100 ? -100 : ex > 100 ? 100 : ex;
+    // Strips whose largest magnitude is below 2^-87 or above 2^113 are out
+    // of the scale's reach (their scaled values would lose precision or
+    // overflow f16): the FP32 form filters them.  u8 input never is.
+    constexpr bool CHECK = !std::is_same_v<IN, uint8_t>;
+    bool redo = CHECK && (ex < -100 || ex > 100);   // wave-uniform
     const float s = ldexpf(a.mq, ex), inv_s = ldexpf(1.f, -ex);
 
     // xh / xl: entries 0 .. 7 (rows 8 g + j), high and low halves; xt (K = 48):
@@ -364,7 +498,10 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // and another convert)
     float one = 1.f;
     asm("" : "+v"(one));
-    auto vpass = [&](int l, vfield& fh, vfield& fl) {
+    // vpass(l, ...) also returns, when `sum` is set, the sum of the field's
+    // values: a NaN or an Inf among the strip's inputs makes it non-finite
+    // (every input of a block meets every band entry in the products).
+    auto vpass = [&](int l, vfield& fh, vfield& fl, bool sum) {
         const int t1 = MBASE + 8 * g - RC - m;
         const v8f16 ch1 = cat8(T.run[(0 * 3 + l) * MT4 + t1], T.run[(0 * 3 + l) * MT4 + t1 + 4]);
         const v8f16 cl1 = cat8(T.run[(1 * 3 + l) * MT4 + t1], T.run[(1 * 3 + l) * MT4 + t1 + 4]);
@@ -393,6 +530,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
         }
 #pragma unroll
         for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xh[b], ch1, acc[b]);
+        float total = 0.f;
+        if (sum) {
+#pragma unroll
+            for (int b = 0; b < NVB; ++b) total += (acc[b][0] + acc[b][1]) + (acc[b][2] + acc[b][3]);
+        }
 #pragma unroll
         for (int b = 0; b < NVB; ++b) {
             v4f16 h, lo;
@@ -403,6 +545,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
 #pragma unroll
             for (int i = 0; i < 4; ++i) { fh[4 * b + i] = h[i]; fl[4 * b + i] = lo[i]; }
         }
+        return total;
     };
 
     // ------------------------------------------------- horizontal pass
@@ -478,14 +621,25 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // One field at a time, each followed by the outputs it feeds, so that
     // the first stores leave after a third of the vertical work.
     const unsigned fneed[3] = {ONE ? 1u : mask & 0x0bu, ONE ? 0u : mask & 0x14u, ONE ? 0u : mask & 0x20u};
+    bool check = CHECK;                         // the first field computed vets the strip's input
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
         if (!fneed[l]) continue;
         vfield fh, fl;
-        vpass(l, fh, fl);
+        const float total = vpass(l, fh, fl, check);
+        if (check) {
+            check = false;
+            redo |= __builtin_amdgcn_ballot_w64(!(__builtin_fabsf(total) <= 3.0e38f)) != 0;
+        }
 #pragma unroll
         for (int k = 0; k < NOUT; ++k)
             if (VL[k] == l && (mask >> k & 1u)) hpass(k, fh, fl);
+    }
+    // Rare: the products could not carry this strip's input.  Its outputs are
+    // written again by the FP32 form, after the stores above have completed.
+    if (redo) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        mfma_strip_fp32<RC, IN, OUT, MODE>(a, y0, x0, T, stage);
     }
 }
 #undef TUCU_MFMA

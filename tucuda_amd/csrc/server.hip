@@ -117,6 +117,7 @@ using Lobes = std::array<float, 3 * 17 + 3>;      // three lobes of 17 + their l
 
 struct tucu_server_s {
     tucu_server_config                     cfg;
+    int                                    fam = 0;      // kernel family of the batched kernel
     int                                    thi = 0;      // tile-height index of the batched kernel
     int                                    device = 0;
     std::chrono::steady_clock::time_point  t0;
@@ -240,7 +241,7 @@ bool reap(tucu_server_s& s)
 // true if a batch was launched.
 bool admit_and_launch(tucu_server_s& s, Slot& sl)
 {
-    const BatchEntry& e = batch_entry(s.thi, s.cfg.in_dtype, s.cfg.out_dtype);
+    const BatchEntry& e = batch_entry(s.fam, s.thi, s.cfg.in_dtype, s.cfg.out_dtype);
     const int tile_w = e.tw ? e.tw : NT - 32;
     std::vector<SepArgs> jobs;
     std::vector<Request*> reqs;
@@ -364,14 +365,18 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
     if (dtype_size(cfg->in_dtype) == 0 || dtype_size(cfg->out_dtype) == 0)
         return fail(TUCU_ERR_INVALID, "unknown dtype");
     int thi = -1;
-    const int rows = cfg->tile_rows > 0 ? cfg->tile_rows : batch_entry(BATCH_TH_DEFAULT, cfg->in_dtype, cfg->out_dtype).th;
+    const int fam = batch_family();
+    const int rows = cfg->tile_rows > 0 ? cfg->tile_rows
+                                        : batch_entry(fam, BATCH_TH_DEFAULT, cfg->in_dtype, cfg->out_dtype).th;
     for (int i = 0; i < NBTH; ++i)
-        if (batch_entry(i, cfg->in_dtype, cfg->out_dtype).th == rows && batch_entry(i, cfg->in_dtype, cfg->out_dtype).fn)
+        if (batch_entry(fam, i, cfg->in_dtype, cfg->out_dtype).th == rows &&
+            batch_entry(fam, i, cfg->in_dtype, cfg->out_dtype).fn)
             thi = i;
     if (thi < 0) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
     if (cfg->pool_bytes == 0) return fail(TUCU_ERR_INVALID, "empty output pool");
     auto s = new tucu_server_s();
     s->cfg = *cfg;
+    s->fam = fam;
     s->thi = thi;
     if (s->cfg.max_batch_jobs <= 0) s->cfg.max_batch_jobs = 1024;
     if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 2;
@@ -430,6 +435,8 @@ int check_request(const tucu_server_s& s, const tucu_request& req)
 {
     if (req.in.dtype != s.cfg.in_dtype) return fail(TUCU_ERR_INVALID, "input dtype differs from the server's");
     if (req.mask == 0 || (req.mask & ~unsigned(TUCU_ALL6))) return fail(TUCU_ERR_INVALID, "bad output mask");
+    if (!(req.sigma > 0.f) || !(req.sigma < 1e6f))                 // the rule tucu_gauss2_lobes applies
+        return fail(TUCU_ERR_INVALID, "sigma must be positive and finite");
     if (!req.in.data || req.in.nrow <= 0 || req.in.ncol <= 0) return fail(TUCU_ERR_INVALID, "bad input image");
     const size_t es = dtype_size(s.cfg.out_dtype);
     const size_t row = round_up(size_t(req.in.ncol) * es, 128);

@@ -365,12 +365,18 @@ __global__ __launch_bounds__(256) void pyramid_kernel(PyrArgs a)
     if (t < NC) {
         const __amdgpu_buffer_rsrc_t rsrc =
             __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
-        const int voff = (xc0 - R + t) * int(sizeof(IN));  // outside the image: 0 or a neighbouring row,
-        float w[NW];                                       // feeding only outputs that are zeroed
+        // Row and column both go into the vector offset, the part of the
+        // address the buffer range check covers (the scalar offset is not
+        // checked): a column left of the image reads the end of the row
+        // above or, on row 0, returns 0; a column right of it reads the
+        // next row or, on the last row, returns 0.  Either feeds only
+        // outputs that are zeroed.
+        const int coff = (xc0 - R + t) * int(sizeof(IN));
+        float w[NW];
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             const int row = min(max(yc0 - R + i, 0), a.nrow - 1);
-            w[i] = Px<IN>::ld(rsrc, voff, row * a.in_stride_b);
+            w[i] = Px<IN>::ld(rsrc, row * a.in_stride_b + coff, 0);
         }
 #pragma unroll
         for (int k = 0; k < PYR_TH; ++k) V[k][t] = pyr_sum<R>(a, &w[2 * k]);
