@@ -44,10 +44,10 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0, budget_1core_s: floa
     import oracle
     import numpy as np
     try:
-        cores = len(os.sched_getaffinity(0))
+        cores = len(os.sched_getaffinity(0))          # every core this process may run on
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = max(1, cores)
     lobes = oracle.gauss_lobes(sigma)
     outs = {m: np.zeros(img.shape, np.float32) for m in oracle.METHOD_ORDER}
 
@@ -68,7 +68,9 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0, budget_1core_s: floa
     n, el = rate(cores, budget_s)
     n1, el1 = rate(1, budget_1core_s)
     return {"value": round(img.size * n / el / 1e9, 5), "unit": "Gpix/s", "cores": cores, "kind": "port",
-            "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads)",
+            "cpu_count": os.cpu_count(),
+            "sample": f"{n} whole 1920x1080 images, 6 outputs each ({el:.1f} s on {cores} threads, "
+                      f"one per core of the process's affinity set)",
             "value_1core": round(img.size * n1 / el1 / 1e9, 5),
             "sample_1core": f"{n1} whole images on 1 thread ({el1:.1f} s)"}
 
@@ -101,37 +103,82 @@ def hot_kernel_us(g, ins, outs, launches: int = 10, reps: int = 20) -> float:
     return e0.elapsed_time(e1) * 1000 / (reps * launches)
 
 
-def write_fill_gbs(dev, nbytes: int, nbuf: int = 10, reps: int = 5) -> float:
-    """HBM write rate of a plain fill of nbytes (one kernel per buffer,
-    nbuf rotating buffers so the Infinity Cache cannot hold them), replayed
-    from a HIP graph: the rate a write stream of the outputs' size reaches
-    with no arithmetic."""
+def _membench():
+    import ctypes
+    path = os.path.join(ROOT, "tucuda_amd", "libtucuda_membench.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (make -C tucuda_amd/csrc)")
+    L = ctypes.CDLL(path)
+    L.tucu_membench_fill.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_void_p]
+    L.tucu_membench_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    return L
+
+
+def write_ceiling(dev, out_bytes: int) -> dict:
+    """Reference rates of the memory system with the filters' store form
+    (16-byte non-temporal stores), from a hand-written fill and copy kernel
+    (tucuda_amd/csrc/membench.hip): the fill of the six outputs' bytes,
+    replayed from a HIP graph over 10 rotating buffers (a launch of this
+    kernel's size, ramp and tail included), and a sustained fill and copy of
+    1 GiB.  GB/s."""
+    import ctypes
     import torch
-    bufs = [torch.empty(nbytes // 4, dtype=torch.float32, device=dev) for _ in range(nbuf)]
-    for b in bufs:
-        b.fill_(0.0)
-    torch.cuda.synchronize()
+    L = _membench()
+    stream = torch.cuda.current_stream(dev)
+    sh = ctypes.c_void_p(stream.cuda_stream)
+
+    def events(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / reps
+
+    nb = (out_bytes + 15) // 16 * 16
+    bufs = [torch.empty(nb // 4, dtype=torch.float32, device=dev) for _ in range(10)]
     graph = torch.cuda.CUDAGraph()
     st = torch.cuda.Stream(dev)
-    st.wait_stream(torch.cuda.current_stream(dev))
+    st.wait_stream(stream)
     with torch.cuda.stream(st):
         with torch.cuda.graph(graph, stream=st):
             for b in bufs:
-                b.fill_(1.0)
-    torch.cuda.current_stream(dev).wait_stream(st)
-    graph.replay()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    rs = torch.cuda.current_stream(dev)
-    e0.record(rs)
-    for _ in range(reps):
-        graph.replay()
-    e1.record(rs)
-    torch.cuda.synchronize()
-    sec = e0.elapsed_time(e1) * 1e-3 / (reps * nbuf)
+                assert L.tucu_membench_fill(ctypes.c_void_p(b.data_ptr()), nb, 1.0,
+                                            ctypes.c_void_p(st.cuda_stream)) == 0
+    stream.wait_stream(st)
+    t_small = events(graph.replay, 20) / len(bufs)
     del graph, bufs
+    big = 1 << 30
+    a = torch.empty(big // 4, dtype=torch.float32, device=dev)
+    b = torch.empty(big // 4, dtype=torch.float32, device=dev)
+    t_fill = events(lambda: L.tucu_membench_fill(ctypes.c_void_p(a.data_ptr()), big, 2.0, sh), 10)
+    t_copy = events(lambda: L.tucu_membench_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
+                                                 big, sh), 10)
+    del a, b
     torch.cuda.empty_cache()
-    return nbytes / sec / 1e9
+    return {"fill_outputs_size_gbs": round(nb / t_small / 1e9, 1), "fill_outputs_size_us": round(t_small * 1e6, 2),
+            "fill_1gib_gbs": round(big / t_fill / 1e9, 1),
+            "copy_1gib_gbs": round(2 * big / t_copy / 1e9, 1),
+            "kernel": "tucuda_amd/csrc/membench.hip: 16-byte non-temporal stores (and loads, copy)"}
+
+
+def batched_leg(reps: int) -> dict:
+    """The batched workload (256 images, log-uniform [240, 2160], sigma
+    U[0.8, 4], radius ceil(4 sigma), six outputs) in one graph-replayed
+    launch, float32 and bfloat16, against the plain per-image fused launches
+    replayed from one captured graph (tucuda_amd.bench_batch)."""
+    from tucuda_amd import bench_batch
+    r = bench_batch.run(images=256, reps=reps, seed=3, per_method_form=False)
+    f32, bf = r["forms"]["float32"], r["forms"]["bfloat16"]
+    return {"workload": r["workload"], "megapixels": r["megapixels"],
+            "f32_batched_ms": f32["batched_ms"], "bf16_batched_ms": bf["batched_ms"],
+            "f32_plain_graph_ms": f32["plain_graph_ms"], "bf16_plain_graph_ms": bf["plain_graph_ms"],
+            "f32_batched_gpix_s": f32["batched_gpix_s"], "bf16_batched_gpix_s": bf["batched_gpix_s"],
+            "bf16_batched_vs_f32_plain_graph": r["bf16_batched_vs_f32_plain_graph"],
+            "tile_rows": {"float32": f32["tile_rows"], "bfloat16": bf["tile_rows"]}}
 
 
 def sigma_sweep(dev, nrow: int, ncol: int, sigmas=(0.8, 1.2, 1.6, 2.0, 2.4, 2.8, 3.2, 4.0)) -> dict:
@@ -218,8 +265,10 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch every step from Python instead of replaying a captured HIP graph")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--serve-seconds", type=float, default=3.0,
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="seconds of Poisson arrivals per served-stream rate (0 skips the served leg)")
+    ap.add_argument("--batch-reps", type=int, default=10,
+                    help="graph replays of the batched leg (0 skips it)")
     ap.add_argument("--serve-rates", default="200,1000,4000", help="served-stream arrival rates, req/s")
     ap.add_argument("--sigma-sweep", action="store_true",
                     help="instead: time one fused six-output 1080p launch for several sigmas "
@@ -396,11 +445,17 @@ def main():
     hot_us = hot_kernel_us(g, *sets[0])
     line["roofline"]["kernel_us_hot_single_buffer"] = round(hot_us, 3)
     log(f"cache-hot (one buffer set): {hot_us:.2f} us per launch")
-    # Context for `frac`: the rate a plain fill of the six outputs' bytes
-    # reaches (the outputs are 85% of the kernel's bytes).
-    fill = write_fill_gbs(dev, 6 * nrow * ncol * 4)
-    line["roofline"]["write_fill_gbs"] = round(fill, 1)
-    log(f"plain fill of {6 * nrow * ncol * 4 / 1e6:.1f} MB: {fill:.0f} GB/s")
+    # Context for `frac`: what a plain write stream of the outputs' size, and
+    # a sustained one, reach (the outputs are 85% of the kernel's bytes).
+    wc = write_ceiling(dev, 6 * (nrow - 32) * (ncol - 32) * 4)
+    line["roofline"]["write_ceiling"] = wc
+    log(f"write ceiling: fill of the outputs' size {wc['fill_outputs_size_gbs']} GB/s "
+        f"({wc['fill_outputs_size_us']} us), 1 GiB fill {wc['fill_1gib_gbs']} GB/s, 1 GiB copy {wc['copy_1gib_gbs']} GB/s")
+    if args.batch_reps > 0:
+        log("batched leg ...")
+        line["batched"] = batched_leg(args.batch_reps)
+        log(f"batched: bf16 {line['batched']['bf16_batched_ms']} ms, f32 plain (graph) "
+            f"{line['batched']['f32_plain_graph_ms']} ms: {line['batched']['bf16_batched_vs_f32_plain_graph']}x")
     if args.serve_seconds > 0:
         line["served"] = served_leg(dev, args.serve_rates, args.serve_seconds)
     if not args.no_cpu:

@@ -123,6 +123,63 @@ class row_iterator
     int            _ncol;
 };
 
+//! A run of `size` rows starting at a row iterator: what cu::make_range and
+//! cu::slice return (TU/cu/iterator.h:568-611), a view of device memory
+//! that owns nothing.  begin() / end() are row iterators, so a range passes
+//! to FIRFilter2::convolve and the Gaussian methods as an Array2 does.
+template <class ITER>
+class range
+{
+  public:
+    using iterator       = ITER;
+    using const_iterator = ITER;
+
+    range(const ITER& begin, int size) : _begin(begin), _size(size) {}
+
+    ITER begin()  const { return _begin; }
+    ITER end()    const { return _begin + _size; }
+    ITER cbegin() const { return begin(); }
+    ITER cend()   const { return end(); }
+    int  size()   const { return _size; }
+    auto operator [](int i) const { return _begin[i]; }
+
+  private:
+    ITER _begin;
+    int  _size;
+};
+
+//! make_range(iter, size) (TU/cu/iterator.h:568-573): `size` rows from iter.
+template <class T> inline range<row_iterator<T>>
+make_range(const row_iterator<T>& iter, int size)
+{
+    return {iter, size};
+}
+
+//! make_range(iter, size, stride, ncol): the same rows read with another
+//! row stride (elements) and row length (TU/cu/iterator.h:575-579).
+template <class T> inline range<row_iterator<T>>
+make_range(const row_iterator<T>& iter, int size, std::ptrdiff_t stride, int ncol)
+{
+    return {row_iterator<T>(iter.get(), stride, ncol), size};
+}
+
+//! slice(iter, idx, size): rows idx .. idx + size - 1 (TU/cu/iterator.h:604-609).
+template <class T> inline range<row_iterator<T>>
+slice(const row_iterator<T>& iter, int idx, int size)
+{
+    return {iter + idx, size};
+}
+
+//! slice(iter, idx, size, jdx, jsize): the sub-image of rows idx .. idx +
+//! size - 1 and columns jdx .. jdx + jsize - 1 (TU/cu/iterator.h:588-609;
+//! the reference itself slices an output this way, FIRFilter.h:286-287).
+template <class T> inline range<row_iterator<T>>
+slice(const row_iterator<T>& iter, int idx, int size, int jdx, int jsize)
+{
+    const row_iterator<T> r = iter + idx;
+    return {row_iterator<T>(r.get() + jdx, r.stride(), jsize), size};
+}
+
 //! The C descriptor of rows [ib, ie).
 template <class T>
 inline tucu_image

@@ -30,7 +30,9 @@ class FIRGaussianConvolver2 : public FIRFilter2<T>
   public:
     enum Method { SMOOTH = 0, DIFF_H, DIFF_V, DIFF_HH, DIFF_HV, DIFF_VV };
 
-    explicit FIRGaussianConvolver2(float sigma = 1.0, int rule = TUCU_RULE_REFERENCE)
+    //! Implicit, as the reference's (FIRGaussianConvolver.h:63):
+    //! FIRGaussianConvolver2<float> g = 2.0f; compiles.
+    FIRGaussianConvolver2(float sigma = 1.0, int rule = TUCU_RULE_REFERENCE)
         : _rule(rule)
     {
         initialize(sigma);

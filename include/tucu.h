@@ -206,6 +206,25 @@ TUCU_API int tucu_subsample(const tucu_image* in, const tucu_image* out, int op,
 TUCU_API int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in,
                                  const tucu_image* out, int shift, void* stream);
 
+/* ---- ICIA's error moment ---------------------------------------------
+ * Replaces ICIA::operator()'s gradient and moment steps (TU/cu/ICIA.h:346-369:
+ * FIRGaussianConvolver2::diffH and diffV into two images, then
+ * cub::DeviceReduce::Sum of ICIAErrorMoment, ICIA.h:61-112): diffH and diffV
+ * of `in` with the Gaussian lobes lobe0 (n0 = r0 + 1 entries) and lobe1 (n1 =
+ * r1) from one fused pass that never writes them, and the sum over every
+ * pixel (row v, column u) of the upper triangle, row by row, of a a^T with
+ * a = image_derivative0(s u, s v, eH, eV) (TU/cu/vec.h:1418-1422 affine, 6
+ * entries, 21 moments; :1285-1290 projective, 8 entries, 36 moments), s = 1 /
+ * max(nrow, ncol), a gradient taken as 0 outside its valid region (the
+ * reference's freshly allocated gradient images hold zeros there).
+ * `moment`: device memory for 21 or 36 floats.  Call first with workspace ==
+ * NULL to get the workspace size in *workspace_bytes (as cub's first call,
+ * ICIA.h:355-360), then with a device buffer at least that large.  A NaN or
+ * an Inf in the image makes the moment non-finite, as the reference's sum. */
+enum { TUCU_ICIA_AFFINE = 6, TUCU_ICIA_PROJECTIVE = 8 };
+TUCU_API int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, const tucu_image* in,
+                              int model, float* moment, void* workspace, size_t* workspace_bytes, void* stream);
+
 /* ---- batched mode (extension) ------------------------------------------
  * Many FIRGaussianConvolver2 jobs of uneven sizes and lobe radii in one
  * persistent launch.  No counterpart in the reference, which filters one
@@ -243,6 +262,7 @@ TUCU_API int tucu_batch_set_tile_rows(tucu_batch batch, int rows);
  * process-wide tuning cache under (dtypes, largest radius, log2 pixels). */
 TUCU_API int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows, float* best_us);
 TUCU_API int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* tile_rows);
+
 /* Tuning cache as JSON lines. */
 TUCU_API int tucu_tune_cache_save(const char* path);
 TUCU_API int tucu_tune_cache_load(const char* path);
@@ -250,8 +270,11 @@ TUCU_API int tucu_tune_cache_load(const char* path);
 /* ---- serving (extension) -------------------------------------------------
  * A continuous-batching server for Gaussian requests, no counterpart in the
  * reference.  Requests are submitted from any thread; a scheduler thread
- * admits them in arrival order while the output pool (a fixed HBM budget,
- * pool_bytes) has room, packs all admitted requests into one batched launch
+ * admits them in arrival order while the output pool has room (pool_bytes
+ * reserved at create; when a request does not fit, the pool grows by
+ * tucu_server_admit_bytes(...) bytes if the device's free HBM less
+ * reserve_bytes holds it within max_pool_bytes, and the request waits for
+ * released outputs otherwise), packs all admitted requests into one batched launch
  * on a free in-flight slot (a stream each, max_inflight of them), so requests
  * that arrive while the GPU works join the next launch; completions are
  * drained with tucu_server_poll and their outputs, which live in the pool,
@@ -270,6 +293,10 @@ typedef struct {
     int32_t max_inflight;     /* launches in flight at once (0: 2) */
     int32_t max_tiles;        /* tiles per launch (0: 2^21) */
     int32_t max_batch_mpix;   /* input megapixels per launch (0: 64); a launch holds at least one request */
+    int32_t reserved0;
+    size_t  max_pool_bytes;   /* the pool grows up to this (0: while free HBM allows; pool_bytes: fixed) */
+    size_t  reserve_bytes;    /* device memory the pool leaves free when it grows (0: 1 GiB) */
+    size_t  grow_bytes;       /* smallest growth step (0: 256 MiB) */
 } tucu_server_config;
 
 typedef struct {
@@ -294,6 +321,13 @@ typedef struct {
     uint64_t pool_bytes, pool_used, pool_peak;
 } tucu_server_counters;
 
+/* The server's admission rule for a request needing `need` bytes of outputs
+ * that its pool (now pool_bytes) cannot place: the bytes to grow the pool by,
+ * max(need, grow_step) clipped to device_free - reserve and to
+ * max_pool_bytes - pool_bytes (max_pool_bytes 0: no cap), or 0 -- wait for
+ * released memory -- when need exceeds either limit.  Host-only. */
+TUCU_API size_t tucu_server_admit_bytes(size_t need, size_t device_free, size_t reserve, size_t pool_bytes,
+                                        size_t max_pool_bytes, size_t grow_step);
 TUCU_API int    tucu_server_create(tucu_server* server, const tucu_server_config* cfg);
 /* Fails requests not yet admitted, waits for in-flight launches, frees everything. */
 TUCU_API int    tucu_server_destroy(tucu_server server);

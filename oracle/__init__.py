@@ -83,6 +83,8 @@ def lib() -> ctypes.CDLL:
         L.orc_fir2_taps_f64.argtypes = [f64p, ctypes.c_int, f64p, ctypes.c_int] + common
         L.orc_subsample_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+        L.orc_icia_moment.argtypes = [f32p, f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      f64p, f64p]
         L.orc_half_to_float.argtypes = [ctypes.c_uint16]
         L.orc_half_to_float.restype = ctypes.c_float
         L.orc_float_to_half.argtypes = [ctypes.c_float]
@@ -260,6 +262,31 @@ def subsample(img: np.ndarray, out: np.ndarray, op) -> np.ndarray:
     if rc != 0:
         raise OracleError(f"oracle returned {rc}")
     return out
+
+
+ICIA_MODELS = {"affine": 6, "projective": 8}
+
+
+def icia_moment(edgeH: np.ndarray, edgeV: np.ndarray, model: str = "affine"):
+    """ICIA's error moment (ICIA.h:61-112, 352-369): the sum over every pixel
+    (row v, column u) of the full-size float gradient images of the upper
+    triangle, row by row, of a a^T with a = image_derivative0(s u, s v, eH,
+    eV) of the affine (vec.h:1418-1422, 6 entries) or projective
+    (vec.h:1285-1290, 8 entries) map, s = 1 / max(nrow, ncol).  Returns
+    (moment, abs_moment) in float64: the sums and the sums of the terms'
+    magnitudes (the scale a summation's rounding error is measured on)."""
+    D = ICIA_MODELS[model]
+    eH = np.ascontiguousarray(edgeH, np.float32)
+    eV = np.ascontiguousarray(edgeV, np.float32)
+    assert eH.shape == eV.shape and eH.ndim == 2
+    nm = D * (D + 1) // 2
+    out, aout = np.zeros(nm), np.zeros(nm)
+    p32, p64 = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+    rc = lib().orc_icia_moment(eH.ctypes.data_as(p32), eV.ctypes.data_as(p32), eH.shape[1],
+                               eH.shape[0], eH.shape[1], D, out.ctypes.data_as(p64), aout.ctypes.data_as(p64))
+    if rc != 0:
+        raise OracleError(f"oracle returned {rc}")
+    return out, aout
 
 
 def full_kernel(lobe) -> np.ndarray:

@@ -26,6 +26,9 @@
 //   TU/cu/fp16.h:55-80                    round-to-nearest-even half conversion
 //   TU/cu/algorithm.h:183-242             subsample: out[y][x] = op at (2y, 2x), early return
 //   TU/cu/functional.h:58-256             the stencil operators and their border guards
+//   TU/cu/ICIA.h:61-112, 352-369          ICIA's error moment over the gradient images
+//   TU/cu/vec.h:1285-1290, 1418-1422      image_derivative0 of the projective / affine maps
+//   TU/cu/array.h:101-118, 319-325        array::ext, the upper triangle of an outer product
 //
 // Status codes: 0 ok, 2 lobe longer than 17, 3 unsupported lobe length,
 // 1 invalid argument.
@@ -404,6 +407,49 @@ void gauss_lobes_ceil4(A sigma, A* lobe0, int* n0, A* lobe1, int* n1, A* lobe2, 
     *n0 = int(s0); *n1 = int(s1); *n2 = int(s2);
 }
 
+// ---------------------------------------------------------------------------
+// ICIA's error moment (TU/cu/ICIA.h:61-112 ICIAErrorMoment, summed by
+// ICIA.h:352-369 with cub::DeviceReduce::Sum over every pixel).  Pixel
+// (v, u) -- row v, column u of the full-size gradient images, whose border
+// the reference leaves at the allocator's zeros -- contributes the upper
+// triangle, row by row, of the outer product a a^T (array::ext,
+// TU/cu/array.h:101-118, 319-325) of
+//   affine      (TU/cu/vec.h:1418-1422): a = (eH u', eH v', eH, eV u', eV v', eV)
+//   projective  (TU/cu/vec.h:1285-1290): a = (eH u', eH v', eH, eV u', eV v', eV,
+//                                           -(eH u' + eV v') u', -(eH u' + eV v') v')
+// with u' = s u, v' = s v, s = 1 / max(nrow, ncol) (ICIA.h:84-89).  Summed
+// in double; `abs_out` receives the sum of the terms' magnitudes (the scale
+// of a summation's rounding error).
+int icia_moment(const float* eH, const float* eV, int64_t stride, int nrow, int ncol, int model,
+                double* out, double* abs_out)
+{
+    if (nrow < 0 || ncol < 0 || (model != 6 && model != 8)) return 1;
+    const int D = model, NM = D * (D + 1) / 2;
+    std::vector<double> acc(NM, 0.0), aacc(NM, 0.0);
+    const double s = double(1.0f / float(std::max(nrow, ncol)));
+    double a[8];
+    for (int v = 0; v < nrow; ++v)
+        for (int u = 0; u < ncol; ++u) {
+            const double h = eH[int64_t(v) * stride + u], w = eV[int64_t(v) * stride + u];
+            const double uf = s * u, vf = s * v;
+            a[0] = h * uf; a[1] = h * vf; a[2] = h;
+            a[3] = w * uf; a[4] = w * vf; a[5] = w;
+            if (D == 8) {
+                const double t = -(h * uf + w * vf);
+                a[6] = t * uf; a[7] = t * vf;
+            }
+            int k = 0;
+            for (int i = 0; i < D; ++i)
+                for (int j = i; j < D; ++j, ++k) {
+                    const double p = a[i] * a[j];
+                    acc[k] += p;
+                    aacc[k] += std::fabs(p);
+                }
+        }
+    for (int k = 0; k < NM; ++k) { out[k] = acc[k]; if (abs_out) abs_out[k] = aacc[k]; }
+    return 0;
+}
+
 }   // namespace
 
 extern "C" {
@@ -476,6 +522,12 @@ int orc_subsample_f32(const void* in, int in_dt, int64_t in_stride, int nrow, in
     return subsample(in, in_dt, in_stride, nrow, ncol, out, out_dt, out_stride, op);
 }
 
+int orc_icia_moment(const float* eH, const float* eV, int64_t stride, int nrow, int ncol, int model,
+                    double* out, double* abs_out)
+{
+    return icia_moment(eH, eV, stride, nrow, ncol, model, out, abs_out);
+}
+
 // Element conversions, exposed so the tests can pin them against numpy.
 float    orc_half_to_float(uint16_t h)   { return half_to_float(h); }
 uint16_t orc_float_to_half(float f)      { return float_to_half(f); }
@@ -484,3 +536,4 @@ uint16_t orc_float_to_bf16(float f)      { return float_to_bf16(f); }
 uint8_t  orc_float_to_u8(float f)        { return to_u8(f); }
 
 }   // extern "C"
+

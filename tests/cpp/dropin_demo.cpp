@@ -123,6 +123,22 @@ main(int argc, char* argv[])
         fir.convolve(inf_d.cbegin(), inf_d.cend(), outf_d.begin());
         save(dir + "/fir_f32.pfm", outf_d);
 
+        // a sub-image through cu::slice (TU/cu/iterator.h:588-609): rows
+        // 10 .. nrow-11, columns 7 .. ncol-14 of the float upload, filtered
+        // into the same window of another image; copy-initialised from a
+        // float, as the reference's implicit constructor allows
+        cu::FIRGaussianConvolver2<float> sub_filter = sigma;
+        const int sr = 10, sc = 7, snr = int(inf_d.nrow()) - 21, snc = int(inf_d.ncol()) - 21;
+        cu::Array2<float> sub_d(inf_d.nrow(), inf_d.ncol());
+        const auto sub_in  = cu::slice(inf_d.cbegin(), sr, snr, sc, snc);
+        const auto sub_out = cu::slice(sub_d.begin(), sr, snr, sc, snc);
+        sub_filter.diffH(sub_in.begin(), sub_in.end(), sub_out.begin());
+        const auto rows = cu::make_range(inf_d.cbegin() + 3, int(inf_d.nrow()) - 6);
+        cu::Array2<float> rows_d(inf_d.nrow(), inf_d.ncol());
+        sub_filter.smooth(rows.begin(), rows.end(), (rows_d.begin() + 3));
+        save(dir + "/slice_diffH.pfm", sub_d);
+        save(dir + "/range_smooth.pfm", rows_d);
+
         // one more subsample operator, on the float upload
         cu::Array2<float> g5(inf_d.nrow() / 2, inf_d.ncol() / 2);
         cu::subsample(inf_d.cbegin(), inf_d.cend(), g5.begin(), cu::gaussian5x5());

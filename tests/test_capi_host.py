@@ -149,3 +149,21 @@ def test_lobe_size_null_is_negative():
     lib = _capi.load()
     assert lib.tucu_lobe_size(None, 0) == -_capi.TUCU_ERR_INVALID
     assert b"null lobe" in lib.tucu_last_error()
+
+
+def test_server_admission_rule():
+    """tucu_server_admit_bytes: grow by max(need, step) clipped to the free
+    device memory less the reserve and to the pool cap; 0 (wait) when the
+    request fits in neither."""
+    lib = _capi.load()
+    f = lib.tucu_server_admit_bytes
+    G, M = 1 << 30, 1 << 20
+    assert f(10 * M, 50 * G, G, 0, 0, 256 * M) == 256 * M          # the growth step
+    assert f(300 * M, 50 * G, G, 0, 0, 256 * M) == 300 * M         # a request larger than the step
+    assert f(10 * M, G + 100 * M, G, 0, 0, 256 * M) == 100 * M     # clipped to free - reserve
+    assert f(200 * M, G + 100 * M, G, 0, 0, 256 * M) == 0          # does not fit: wait
+    assert f(10 * M, 500 * M, G, 0, 0, 256 * M) == 0               # less free than the reserve
+    assert f(10 * M, 50 * G, G, 4 * G, 4 * G + 64 * M, 256 * M) == 64 * M   # clipped to the cap
+    assert f(100 * M, 50 * G, G, 4 * G, 4 * G + 64 * M, 256 * M) == 0       # beyond the cap
+    assert f(10 * M, 50 * G, G, 4 * G, 4 * G, 256 * M) == 0                 # a fixed pool
+    assert f(0, 50 * G, G, 0, 0, 256 * M) == 0

@@ -85,3 +85,35 @@ def test_dropin_demo(dev, tmp_path):
     ref = np.zeros((75, 130), np.float32)
     oracle.subsample(img.astype(np.float32), ref, "gaussian5x5")
     np.testing.assert_array_equal(read_pnm(tmp_path / "gaussian5x5_half.pfm"), ref)
+
+
+def test_dropin_slice_and_make_range(dev, tmp_path):
+    """cu::slice(it, 10, nrow-21, 7, ncol-21) and cu::make_range(it + 3,
+    nrow - 6) as inputs and outputs: each result equals the oracle on the
+    sub-image, placed in the same window, and nothing outside it is
+    written."""
+    assert os.path.exists(DEMO), "build the C++ demo first (__graft_entry__.build())"
+    sigma = 1.5
+    img = synth.u8_image(22, 120, 200)
+    pgm = tmp_path / "in.pgm"
+    with open(pgm, "wb") as f:
+        f.write(b"P5\n200 120\n255\n" + img.tobytes())
+    out = subprocess.run([DEMO, str(pgm), str(sigma), "8", str(tmp_path), "1"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    f = img.astype(np.float32)
+    lobes = oracle.gauss_lobes(sigma)
+    got = read_pnm(tmp_path / "slice_diffH.pfm")
+    sub = np.ascontiguousarray(f[10:10 + 120 - 21, 7:7 + 200 - 21])
+    ref = np.zeros(sub.shape, np.float32)
+    oracle.gauss("diffH", sigma, sub, ref, lobes=lobes)
+    assert_f32_parity(got[10:10 + sub.shape[0], 7:7 + sub.shape[1]], ref, "slice diffH")
+    outside = np.ones(got.shape, bool)
+    outside[10:10 + sub.shape[0], 7:7 + sub.shape[1]] = False
+    assert (got[outside] == 0).all()
+    got = read_pnm(tmp_path / "range_smooth.pfm")
+    rows = np.ascontiguousarray(f[3:120 - 3])
+    ref = np.zeros(rows.shape, np.float32)
+    oracle.gauss("smooth", sigma, rows, ref, lobes=lobes)
+    assert_f32_parity(got[3:120 - 3], ref, "make_range smooth")
+    assert (got[:3] == 0).all() and (got[120 - 3:] == 0).all()

@@ -1,0 +1,103 @@
+"""ICIA's error moment (ICIA.h:346-369): the fused diffH / diffV pass that
+feeds the moment reduction without writing the gradients, against the
+oracle -- the FP32-faithful gradients of FIRGaussianConvolver2 in
+zero-filled images, summed in FP64 (oracle.icia_moment, pinned by closed
+forms in tests/test_oracle_icia.py).  Tolerance: each moment within 5e-5 of
+the sum of its terms' magnitudes (the gradients agree to about 1e-6
+relative; a product of two doubles that; the sums' order differs)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from tests._util import to_dev
+from tucuda_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_moment(img, sigma, model):
+    lobes = oracle.gauss_lobes(sigma)
+    eH = np.zeros(img.shape, np.float32)
+    eV = np.zeros(img.shape, np.float32)
+    oracle.gauss("diffH", sigma, img, eH, lobes=lobes)
+    oracle.gauss("diffV", sigma, img, eV, lobes=lobes)
+    return oracle.icia_moment(eH, eV, model)
+
+
+@pytest.mark.parametrize("model", ["affine", "projective"])
+@pytest.mark.parametrize("sigma", [1.0, 2.0, 3.0])
+@pytest.mark.parametrize("shape", [(120, 200), (257, 131)])
+def test_icia_moment_matches_oracle(dev, model, sigma, shape):
+    import torch
+    import tucuda_amd
+    img = synth.edge_image(int(sigma * 10) + shape[0], *shape)
+    got = tucuda_amd.icia_moment(sigma, to_dev(img, dev), model).cpu().numpy().astype(np.float64)
+    torch.cuda.synchronize()
+    want, absm = _oracle_moment(img, sigma, model)
+    assert got.shape == want.shape
+    np.testing.assert_array_less(np.abs(got - want), 5e-5 * absm + 1e-30)
+
+
+def test_icia_moment_full_hd_and_u8(dev):
+    import torch
+    import tucuda_amd
+    sigma = 3.0
+    img = synth.edge_image(5, 1080, 1920)
+    got = tucuda_amd.icia_moment(sigma, to_dev(img, dev), "projective").cpu().numpy().astype(np.float64)
+    want, absm = _oracle_moment(img, sigma, "projective")
+    np.testing.assert_array_less(np.abs(got - want), 5e-5 * absm)
+    img8 = synth.u8_image(6, 300, 420)
+    got = tucuda_amd.icia_moment(2.0, to_dev(img8, dev), "affine").cpu().numpy().astype(np.float64)
+    want, absm = _oracle_moment(img8, 2.0, "affine")
+    np.testing.assert_array_less(np.abs(got - want), 5e-5 * absm)
+    torch.cuda.synchronize()
+
+
+def test_icia_moment_of_the_fused_gradients(dev):
+    """The moment kernel sums exactly the gradients the fused six-output
+    launch writes: the oracle's moment of those images agrees to 1e-5."""
+    import torch
+    import tucuda_amd
+    sigma = 2.5
+    img = synth.edge_image(8, 333, 517)
+    x = to_dev(img, dev)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    eH, eV = torch.zeros_like(x), torch.zeros_like(x)
+    g.convolve6(x, {"diffH": eH, "diffV": eV})
+    got = tucuda_amd.icia_moment(sigma, x, "affine").cpu().numpy().astype(np.float64)
+    want, absm = oracle.icia_moment(eH.cpu().numpy(), eV.cpu().numpy(), "affine")
+    np.testing.assert_array_less(np.abs(got - want), 1e-5 * absm)
+
+
+def test_icia_moment_small_image_and_workspace(dev):
+    """An image too small for the lobes leaves both gradients zero (the
+    reference's convolve returns early): the moment is exactly 0.  The
+    workspace query and a too-small workspace."""
+    import torch
+    import tucuda_amd
+    from tucuda_amd import _capi
+    from tucuda_amd.filters import image_desc
+    x = torch.ones((20, 20), device=dev)
+    got = tucuda_amd.icia_moment(3.0, x, "affine")
+    assert (got.cpu().numpy() == 0).all()
+    l0, l1, _ = tucuda_amd.gauss_lobes(2.0)
+    fp = ctypes.POINTER(ctypes.c_float)
+    y = torch.ones((200, 300), device=dev)
+    src = image_desc(y)
+    need = ctypes.c_size_t(0)
+    lib = _capi.load()
+    assert lib.tucu_icia_moment(l0.ctypes.data_as(fp), l0.size, l1.ctypes.data_as(fp), l1.size, ctypes.byref(src),
+                                6, None, None, ctypes.byref(need), None) == 0
+    assert need.value > 0
+    small = ctypes.c_size_t(need.value - 256)
+    ws = torch.empty(need.value, dtype=torch.uint8, device=dev)
+    mom = torch.empty(21, device=dev)
+    rc = lib.tucu_icia_moment(l0.ctypes.data_as(fp), l0.size, l1.ctypes.data_as(fp), l1.size, ctypes.byref(src),
+                              6, ctypes.c_void_p(mom.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                              ctypes.byref(small), None)
+    assert rc == _capi.TUCU_ERR_INVALID
+    # a constant image has zero gradients everywhere: the moment is 0
+    got = tucuda_amd.icia_moment(2.0, y, "projective")
+    assert float(got.abs().max()) == 0.0

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._util import REL_TOL, assert_f32_parity, assert_u8_parity, to_dev, to_host
+from tests._util import REL_TOL, assert_f32_parity, assert_u8_parity, random_lobe, to_dev, to_host
 from tucuda_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -85,8 +85,8 @@ def test_fir2_nonfinite_masks(dev, kernel_family, nH, nV):
     import torch
     import tucuda_amd
     rng = np.random.default_rng(nH * 31 + nV)
-    lobeH = rng.standard_normal(nH).astype(np.float32)
-    lobeV = rng.standard_normal(nV).astype(np.float32)
+    lobeH = random_lobe(rng, nH)                     # unit L1 gain, so the FP32 bar applies as is
+    lobeV = random_lobe(rng, nV)
     lobeH[1] = 0.0                                   # a zero tap inside the lobe still multiplies
     img = _planted(13, 70, 130)
     f = tucuda_amd.FIRFilter2().initialize(lobeH, lobeV)
@@ -114,8 +114,9 @@ def test_general_taps_nonfinite_masks(dev, nH, nV):
     import torch
     import tucuda_amd
     rng = np.random.default_rng(nH + 100 * nV)
-    th = rng.standard_normal(nH).astype(np.float32)
-    tv = rng.standard_normal(nV).astype(np.float32)
+    th = rng.standard_normal(nH)
+    tv = rng.standard_normal(nV)
+    th, tv = (th / np.abs(th).sum()).astype(np.float32), (tv / np.abs(tv).sum()).astype(np.float32)
     th[nH // 2 - 1] = 0.0
     img = _planted(14, 75, 140)
     f = tucuda_amd.SeparableFilter2(th, tv)

@@ -130,7 +130,7 @@ def test_pool_backpressure(dev):
     reqs = _requests(dev, 12, 33, torch.bfloat16, sizes=(200, 300))
     per_req = max(6 * ((x.shape[1] * 2 + 127) // 128 * 128) * x.shape[0] for x, *_ in reqs)
     pool = 2 * per_req + 4096
-    with Server(pool) as srv:
+    with Server(pool, max_pool_bytes=pool) as srv:           # a fixed pool: no growth
         ids = [srv.submit(x, s, 0x3F) for x, s, _, _ in reqs]
         done = {}
         t_end = time.monotonic() + 60
@@ -186,7 +186,7 @@ def test_server_errors(dev):
     import tucuda_amd
     from tucuda_amd.serve import Server
     x = torch.zeros((64, 64), dtype=torch.bfloat16, device=dev)
-    with Server(1 << 20) as srv:
+    with Server(1 << 20, max_pool_bytes=1 << 20) as srv:
         with pytest.raises(tucuda_amd.TucuError):
             srv.submit(x.float(), 2.0, 1)                     # dtype differs from the server's
         with pytest.raises(tucuda_amd.TucuError):
@@ -195,7 +195,7 @@ def test_server_errors(dev):
             srv.submit(x, 2.0, 1 << 6)                        # unknown output
         big = torch.zeros((1024, 1024), dtype=torch.bfloat16, device=dev)
         with pytest.raises(tucuda_amd.TucuError):
-            srv.submit(big, 2.0, 0x3F)                        # larger than the pool
+            srv.submit(big, 2.0, 0x3F)                        # larger than the pool may grow
         with pytest.raises(tucuda_amd.TucuError):
             srv.release(12345)                                # unknown id
         # an image too small for the filter completes with nothing written
@@ -216,7 +216,7 @@ def test_release_refuses_unfinished(dev):
     import tucuda_amd
     from tucuda_amd.serve import Server
     x = torch.zeros((256, 512), dtype=torch.bfloat16, device=dev)
-    with Server(1 << 20) as srv:                              # 1 MiB of outputs
+    with Server(1 << 20, max_pool_bytes=1 << 20) as srv:     # 1 MiB of outputs, fixed
         ra = srv.submit(x, 2.0, 0x03)                         # 2 outputs: 512 KiB
         _wait(srv, [ra])
         rb = srv.submit(x, 2.0, 0x07)                         # 3 outputs: 768 KiB, waits for ra's memory
@@ -258,3 +258,34 @@ def test_submit_many(dev):
             srv.release(rid)
         assert srv.stats()["pool_used"] == 0
         assert srv.submit_many([]) == []
+
+
+def test_pool_grows_by_free_hbm(dev):
+    """A pool that starts empty grows in chunks while the device's free
+    memory (less the reserve) allows; a reserve larger than the device keeps
+    it from growing, and a request that then can never fit fails with
+    TUCU_ERR_BAD_ALLOC instead of waiting forever."""
+    import torch
+    import tucuda_amd
+    from tucuda_amd.serve import Server
+    reqs = _requests(dev, 6, 71, torch.bfloat16, sizes=(300, 400))
+    with Server(0, grow_bytes=1 << 20) as srv:
+        assert srv.stats()["pool_bytes"] == 0
+        ids = [srv.submit(x, s, 0x3F) for x, s, _, _ in reqs]
+        got = _wait(srv, ids)
+        st = srv.stats()
+        assert st["pool_bytes"] >= st["pool_peak"] > 0
+        for rid, (x, sigma, mask, _) in zip(ids, reqs):
+            c = got[rid]
+            assert c["status"] == 0
+            outs = _outputs(srv, c, x, 0x3F)
+            ref = _single(x, sigma, 0x3F)
+            for k in outs:
+                _same(outs[k], ref[k], f"req {rid} out {k}")
+            srv.release(rid)
+    total = torch.cuda.get_device_properties(dev).total_memory
+    with Server(0, reserve_bytes=2 * total) as srv:
+        rid = srv.submit(reqs[0][0], 2.0, 0x01)
+        c = _wait(srv, [rid])[rid]
+        assert c["status"] == 4                               # TUCU_ERR_BAD_ALLOC
+        assert srv.stats()["pool_bytes"] == 0

@@ -16,7 +16,8 @@ def __getattr__(name):
     # torch is imported lazily so that the ABI can be inspected without it.
     if name in ("FIRFilter2", "FIRGaussianConvolver2", "SeparableFilter2", "GaussBatch", "gauss_lobes",
                 "lobe_size", "tune_cache_save", "tune_cache_load",
-                "METHOD_ORDER", "METHOD_BITS", "METHOD_LOBES", "image_desc", "batch_tile_rows", "array2", "subsample", "SUBSAMPLE_OPS"):
+                "METHOD_ORDER", "METHOD_BITS", "METHOD_LOBES", "image_desc", "batch_tile_rows", "array2", "subsample",
+                "SUBSAMPLE_OPS", "icia_moment", "ICIA_MODELS"):
         from . import filters
         return getattr(filters, name)
     raise AttributeError(name)

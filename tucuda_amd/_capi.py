@@ -21,6 +21,7 @@ TUCU_ERR_DEVICE = 5
 TUCU_U8, TUCU_F32, TUCU_F16, TUCU_BF16 = 0, 1, 2, 3
 TUCU_RULE_REFERENCE, TUCU_RULE_CEIL4SIGMA = 0, 1
 TUCU_ALL6 = 0x3F
+TUCU_ICIA_AFFINE, TUCU_ICIA_PROJECTIVE = 6, 8
 
 # cu::subsample operators (functional.h:58-256), TUCU_OP_* in include/tucu.h.
 SUBSAMPLE_OPS = ("identity1x1", "diffH1x3", "diffV3x1", "diffHH1x3", "diffVV3x1", "diffHV3x3",
@@ -32,9 +33,11 @@ EXPORTS = (
     "tucu_memcpy2d", "tucu_synchronize", "tucu_fir2_check", "tucu_fir2_convolve",
     "tucu_fir2_convolve_taps",
     "tucu_lobe_size", "tucu_subsample", "tucu_gauss2_pyramid", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
+    "tucu_icia_moment",
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
-    "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save", "tucu_tune_cache_load",
-    "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
+    "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save",
+    "tucu_tune_cache_load",
+    "tucu_server_admit_bytes", "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
     "tucu_server_submit_many", "tucu_server_poll", "tucu_server_release", "tucu_server_stats",
 )
 
@@ -76,6 +79,10 @@ class TucuServerConfig(ctypes.Structure):
         ("max_inflight", ctypes.c_int32),
         ("max_tiles", ctypes.c_int32),
         ("max_batch_mpix", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("max_pool_bytes", ctypes.c_size_t),
+        ("reserve_bytes", ctypes.c_size_t),
+        ("grow_bytes", ctypes.c_size_t),
     ]
 
 
@@ -177,6 +184,10 @@ def load() -> ctypes.CDLL:
     L.tucu_gauss2_convolve.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, fp, ctypes.c_int,
                                        imgp, imgp, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]
     vp = ctypes.c_void_p
+    L.tucu_icia_moment.argtypes = [fp, ctypes.c_int, fp, ctypes.c_int, imgp, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), ctypes.c_void_p]
+    L.tucu_server_admit_bytes.argtypes = [ctypes.c_size_t] * 6
+    L.tucu_server_admit_bytes.restype = ctypes.c_size_t
     L.tucu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TucuGaussJob), ctypes.c_int]
     L.tucu_batch_destroy.argtypes = [vp]
     L.tucu_batch_launch.argtypes = [vp, vp]
@@ -196,7 +207,8 @@ def load() -> ctypes.CDLL:
     L.tucu_server_release.argtypes = [vp, ctypes.c_uint64]
     L.tucu_server_stats.argtypes = [vp, ctypes.POINTER(TucuServerCounters)]
     for name in EXPORTS:
-        if name not in ("tucu_last_error", "tucu_row_stride", "tucu_version", "tucu_server_now"):
+        if name not in ("tucu_last_error", "tucu_row_stride", "tucu_version", "tucu_server_now",
+                        "tucu_server_admit_bytes"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

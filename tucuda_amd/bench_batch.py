@@ -55,12 +55,24 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--dtypes", default="float32,bfloat16")
     ap.add_argument("--packed", action="store_true", help="rows packed at their width instead of a 256-byte pitch")
+    ap.add_argument("--no-per-method", action="store_true", help="skip the method-per-output form")
     args = ap.parse_args(argv)
+    print(json.dumps(run(args.images, args.reps, args.seed, args.dtypes, args.packed, not args.no_per_method)),
+          flush=True)
 
+
+def run(images: int = 256, reps: int = 20, seed: int = 3, dtypes: str = "float32,bfloat16",
+        packed: bool = False, per_method_form: bool = True) -> dict:
+    """The measurement described in the module docstring; returns the dict
+    main() prints."""
     import torch
     import tucuda_amd
     from . import synth
 
+    class _Args:
+        pass
+    args = _Args()
+    args.images, args.reps, args.seed, args.dtypes, args.packed = images, reps, seed, dtypes, packed
     dev = torch.device("cuda:0")
     sizes = make_sizes(args.images, args.seed)
     total_px = sum(h * w for h, w, _ in sizes)
@@ -141,7 +153,7 @@ def main(argv=None):
             "tile_rows": rows, "tiles": batch.info()["tiles"], "grid": batch.info()["grid"],
             "table_build_s": round(t_build, 3),
         }
-        if dname == "float32":
+        if dname == "float32" and per_method_form:
             ref_gs = [tucuda_amd.FIRGaussianConvolver2(sg) for _, _, sg in sizes]
 
             def per_method():
@@ -167,7 +179,7 @@ def main(argv=None):
         if "per_method_ms" in f32:
             result["bf16_batched_vs_f32_per_method"] = round(f32["per_method_ms"] / bf["batched_ms"], 2)
             result["bf16_batched_vs_f32_per_method_graph"] = round(f32["per_method_graph_ms"] / bf["batched_ms"], 2)
-    print(json.dumps(result), flush=True)
+    return result
 
 
 if __name__ == "__main__":

@@ -165,6 +165,40 @@ int launch_layout(tucu_batch_s& b, int thi, hipStream_t s)
     return e == hipSuccess ? TUCU_OK : device_fail(e, "batched kernel launch");
 }
 
+// Kernel arguments of every job that writes something, validated as
+// tucu_gauss2_convolve validates one call; all jobs share one input and one
+// output element type, and none filters in place.  rclass: the largest
+// radius class; work: centre pixels covered.
+int batch_jobs(const tucu_gauss_job* jobs, int njobs, std::vector<SepArgs>& out, int& in_dt, int& out_dt,
+               int& rclass, int64_t& work)
+{
+    if (njobs < 0 || (njobs > 0 && !jobs)) return fail(TUCU_ERR_INVALID, "bad job array");
+    out.clear();
+    rclass = 0;
+    work = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const tucu_gauss_job& g = jobs[j];
+        const float* lobes[3] = {g.lobe0, g.lobe1, g.lobe2};
+        const int n[3] = {g.n0, g.n1, g.n2};
+        SepArgs a;
+        int rmax = 0, odt = 0;
+        const int rc = gauss_args(lobes, n, &g.in, g.out, g.mask, g.shift, a, rmax, odt);
+        if (rc) return rc;
+        if (!a.mask) continue;                                  // nothing to write for this job
+        if (const int al = gauss_aliasing(&g.in, g.out, a.mask, a.r, g.shift != 0))
+            return al == -1 ? fail(TUCU_ERR_INVALID, "job " + std::to_string(j) +
+                                                     ": an output overlaps the input (batched jobs cannot filter in place)")
+                            : al;
+        if (out.empty()) { in_dt = g.in.dtype; out_dt = odt; }
+        else if (g.in.dtype != in_dt || odt != out_dt)
+            return fail(TUCU_ERR_INVALID, "all jobs of a batch must share input and output element types");
+        out.push_back(a);
+        rclass = std::max(rclass, a.rt);
+        work += int64_t(a.cy1 - a.cy0) * (a.cx1 - a.cx0);
+    }
+    return TUCU_OK;
+}
+
 }   // namespace
 
 extern "C" {
@@ -173,36 +207,11 @@ int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
 {
     if (!batch) return fail(TUCU_ERR_INVALID, "null batch handle");
     *batch = nullptr;
-    if (njobs < 0 || (njobs > 0 && !jobs)) return fail(TUCU_ERR_INVALID, "bad job array");
     auto b = new tucu_batch_s();
     b->fam = batch_family();
-    int rmax_all = 0;
     int64_t work = 0;
-    for (int j = 0; j < njobs; ++j) {
-        const tucu_gauss_job& g = jobs[j];
-        const float* lobes[3] = {g.lobe0, g.lobe1, g.lobe2};
-        const int n[3] = {g.n0, g.n1, g.n2};
-        SepArgs a;
-        int rmax = 0, odt = 0;
-        const int rc = gauss_args(lobes, n, &g.in, g.out, g.mask, g.shift, a, rmax, odt);
-        if (rc) { delete b; return rc; }
-        if (!a.mask) continue;                                  // nothing to write for this job
-        if (const int al = gauss_aliasing(&g.in, g.out, a.mask, a.r, g.shift != 0)) {
-            delete b;
-            return al == -1 ? fail(TUCU_ERR_INVALID, "job " + std::to_string(j) +
-                                                     ": an output overlaps the input (batched jobs cannot filter in place)")
-                            : al;
-        }
-        if (b->jobs.empty()) { b->in_dt = g.in.dtype; b->out_dt = odt; }
-        else if (g.in.dtype != b->in_dt || odt != b->out_dt) {
-            delete b;
-            return fail(TUCU_ERR_INVALID, "all jobs of a batch must share input and output element types");
-        }
-        b->jobs.push_back(a);
-        rmax_all = std::max(rmax_all, a.rt);
-        work += int64_t(a.cy1 - a.cy0) * (a.cx1 - a.cx0);
-    }
-    b->rclass = rmax_all;
+    const int rc = batch_jobs(jobs, njobs, b->jobs, b->in_dt, b->out_dt, b->rclass, work);
+    if (rc) { delete b; return rc; }
     b->bucket = work > 0 ? int(std::log2(double(work))) : 0;
     if (hipMalloc(&b->d_counter, QUEUE_BYTES) != hipSuccess) {
         (void)hipGetLastError();

@@ -26,6 +26,7 @@ KernelEntry g_taps_table[4][4][4][4];
 KernelEntry g_gauss_table[5][NGTH][4][4];
 KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 KernelEntry g_fir_mfma_table[2][4][4];
+IciaLaunchFn g_icia_table[2][4][2];
 SubLaunchFn g_sub_table[NSUBOPS][4][4];
 PyrLaunchFn g_pyr_table[3][4][4];
 
@@ -746,6 +747,70 @@ int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* in, const 
     if (!fn) return copy.finish(fail(TUCU_ERR_INVALID, "no kernel for this type combination"), st);
     const hipError_t e = fn(a, st);
     return copy.finish(e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch"), st);
+}
+
+int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, const tucu_image* in,
+                     int model, float* moment, void* workspace, size_t* workspace_bytes, void* stream)
+{
+    int rc;
+    if (model != TUCU_ICIA_AFFINE && model != TUCU_ICIA_PROJECTIVE) return fail(TUCU_ERR_INVALID, "unknown ICIA model");
+    if (!workspace_bytes) return fail(TUCU_ERR_INVALID, "null workspace size");
+    const int r0 = n0 - 1, r1 = n1;                  // lobe 0 symmetric (r + 1 entries), lobe 1 antisymmetric (r)
+    if (r0 < 1 || r0 > 16 || r1 < 1 || r1 > 16)
+        return fail(TUCU_ERR_UNSUPPORTED_LOBE, "lobe length outside the supported range");
+    if (!lobe0 || !lobe1) return fail(TUCU_ERR_INVALID, "null lobe");
+    if ((rc = check_image(in, "input", false))) return rc;
+    const int nrow = in->nrow, ncol = in->ncol;
+    SepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    // diffH = (H lobe 1, V lobe 0) is valid on rows [r0, nrow - r0), columns
+    // [r1, ncol - r1); diffV = (H lobe 0, V lobe 1) on rows [r1, nrow - r1),
+    // columns [r0, ncol - r0) (FIRFilter.h:37-40, 286-287); an image too small
+    // for one of them leaves it all zero (FIRFilter.h:304-312).
+    unsigned live = 0;
+    int cy0 = nrow, cy1 = 0, cx0 = ncol, cx1 = 0;
+    if (nrow > 2 * r0 && ncol > 2 * r1) {
+        live |= TUCU_DIFFH;
+        cy0 = std::min(cy0, r0); cy1 = std::max(cy1, nrow - r0); cx0 = std::min(cx0, r1); cx1 = std::max(cx1, ncol - r1);
+    }
+    if (nrow > 2 * r1 && ncol > 2 * r0) {
+        live |= TUCU_DIFFV;
+        cy0 = std::min(cy0, r1); cy1 = std::max(cy1, nrow - r1); cx0 = std::min(cx0, r0); cx1 = std::max(cx1, ncol - r0);
+    }
+    const int D = model, NM = D * (D + 1) / 2;
+    constexpr int WY = MFMA_WYS[MFMA_WY_DEFAULT];
+    int64_t nblocks = 0;
+    if (live) {
+        a.tiles_y = (cy1 - cy0 + 16 * WY - 1) / (16 * WY);
+        nblocks = int64_t((cx1 - cx0 + MSTRIP_W * (MWAVES / WY) - 1) / (MSTRIP_W * (MWAVES / WY))) * a.tiles_y;
+        if (nblocks > (int64_t(1) << 31) - 1) return fail(TUCU_ERR_INVALID, "image too large");
+    }
+    const size_t need = (size_t(nblocks) * size_t(NM) * sizeof(double) + 255) / 256 * 256 + 256;
+    if (!workspace) { *workspace_bytes = need; return TUCU_OK; }  // the size query (cub's two-call form, ICIA.h:355-366)
+    if (*workspace_bytes < need) return fail(TUCU_ERR_INVALID, "workspace smaller than tucu_icia_moment asked for");
+    if (!moment) return fail(TUCU_ERR_INVALID, "null moment");
+    if (live && !in->data) return fail(TUCU_ERR_INVALID, "input: null data");
+    if (live) {
+        const int64_t es = int64_t(dtype_size(in->dtype));
+        const int64_t stride_b = in->stride * es;
+        if (int64_t(nrow) * stride_b > INT32_MAX) return fail(TUCU_ERR_INVALID, "input images are limited to 2 GiB");
+        a.in = in->data;
+        a.in_stride_b = int32_t(stride_b);
+        a.in_bytes = uint32_t(int64_t(nrow - 1) * stride_b + ncol * es);
+        a.nrow = nrow; a.ncol = ncol;
+        a.cy0 = cy0; a.cy1 = cy1; a.cx0 = cx0; a.cx1 = cx1;
+        a.r[0] = r0; a.r[1] = r1;
+        centre_index(lobe0, r0, true, a.cv[0]);
+        centre_index(lobe1, r1, false, a.cv[1]);
+        mfma_scales(a);
+        a.mask = live;
+        a.rt = std::max(r0, r1) <= 8 ? 8 : 16;
+    }
+    double* partial = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(workspace) + 255) / 256 * 256);
+    const IciaLaunchFn fn = g_icia_table[std::max(r0, r1) <= 8 ? 0 : 1][in->dtype][model == TUCU_ICIA_AFFINE ? 0 : 1];
+    if (!fn) return fail(TUCU_ERR_INVALID, "no ICIA kernel for this input type");
+    const hipError_t e = fn(a, int(nblocks), partial, moment, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? TUCU_OK : device_fail(e, "kernel launch");
 }
 
 }   // extern "C"

@@ -292,7 +292,11 @@ mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage)
     constexpr bool ONE = MODE == MODE_TAPS;
     constexpr int NFC = 32 + 2 * RC;            // field columns per half strip
     static_assert(NFC <= MOPITCH, "staging area too narrow");
-    const int lane = threadIdx.x & 63;
+    // Values the compiler cannot see through: nothing of this rarely taken
+    // path is computed ahead of it (in a tile loop, such hoisted values would
+    // hold registers across the matrix-core path).
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane), "+s"(y0), "+s"(x0));
     const int es = int(sizeof(IN));
     const int sb = a.in_stride_b;
     const unsigned mask = a.mask;
@@ -393,6 +397,16 @@ typedef float MfmaRaw[6][12];
 // needs more registers.
 template <class IN> constexpr int MFMA_WAVES_PER_SIMD = sizeof(IN) == 4 ? 2 : 3;
 
+// The single-image tile loop (large images): two waves per SIMD for the
+// six-output Gaussian (the strip code inlined in a loop needs more
+// registers); the one-output mode (FIRFilter2 lobes, general taps) needs
+// fewer, but at three or four waves per SIMD it spills (176-236 bytes per
+// lane at four): two by default, TUCU_LOOP_WAVES_TAPS at build time to try.
+#ifndef TUCU_LOOP_WAVES_TAPS
+#define TUCU_LOOP_WAVES_TAPS 2
+#endif
+template <int MODE, class IN> constexpr int MFMA_LOOP_WAVES = MODE == MODE_TAPS ? TUCU_LOOP_WAVES_TAPS : 2;
+
 // Issues the loads of the strip at (y0, x0); the values arrive in raw.
 template <int RC, class IN, class A>
 __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& raw)
@@ -439,9 +453,10 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
 //   horizontal, K48: two K32 slices over the fixed pairs P_(o/2) and
 //     P_(o/2 + 1) of field blocks (2 p, 2 p + 1); A = taps as above, shifted
 //     by d = 32 p - 16 o (zero for the block outside the window).
-template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
+template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F, class S>
 __device__ __forceinline__ void
-mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
+mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass,
+             S&& sink)
 {
     constexpr bool ONE = MODE == MODE_TAPS;     // one field (table 0), one output (table 1 across)
     using G = MGeo<RC>;
@@ -466,7 +481,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // Strips whose largest magnitude is below 2^-87 or above 2^113 are out
     // of the scale's reach (their scaled values would lose precision or
     // overflow f16): the FP32 form filters them.  u8 input never is.
-    constexpr bool CHECK = !std::is_same_v<IN, uint8_t>;
+    constexpr bool CHECK = !std::is_same_v<IN, uint8_t> && MODE != MODE_ICIA;
     bool redo = CHECK && (ex < -100 || ex > 100);   // wave-uniform
     const float s = ldexpf(a.mq, ex), inv_s = ldexpf(1.f, -ex);
 
@@ -593,6 +608,10 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
             static_for<0, MNB>([&](auto oc) { constexpr int o = oc; acc[o] = TUCU_MFMA(ch1, pair8<o>(fh), acc[o]); });
         }
         const float u = a.mu[k] * inv_s;
+        if constexpr (MODE == MODE_ICIA) {      // the gradient stays on chip
+            sink(k, acc, u);
+            return;
+        }
         if (fast) {
             // Through the wave's LDS staging area, so that each store writes
             // 4 rows x 64 contiguous pixels instead of 16 rows x 16.
@@ -637,10 +656,20 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     }
     // Rare: the products could not carry this strip's input.  Its outputs are
     // written again by the FP32 form, after the stores above have completed.
-    if (redo) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        mfma_strip_fp32<RC, IN, OUT, MODE>(a, y0, x0, T, stage);
+    if constexpr (CHECK) {
+        if (redo) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            mfma_strip_fp32<RC, IN, OUT, MODE>(a, y0, x0, T, stage);
+        }
     }
+}
+
+// The same without a sink (every mode that writes its outputs).
+template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
+__device__ __forceinline__ void
+mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
+{
+    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T, stage, after_vpass, 0);
 }
 #undef TUCU_MFMA
 
@@ -703,7 +732,7 @@ sepfir_mfma_kernel(const SepArgs a)
 // them.  (A 4096 x 4096 FIRFilter2 launch: 21.5 us against 25.5 us with one
 // workgroup per tile, which rebuilds the tables 4096 times.)
 template <int RC, int WY, class IN, class OUT, int MODE = MODE_GAUSS>
-__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(MFMA_LOOP_WAVES<MODE, IN>)))
 sepfir_mfma_loop_kernel(const SepArgs a, int ntiles)
 {
     __shared__ MfmaTabs T;
@@ -869,7 +898,137 @@ constexpr int MFMA_RADII[2] = {8, 16};
 constexpr int NMWY = 3;
 constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
 constexpr int MFMA_WY_DEFAULT = 1;          // index: 2 x 2 strips, 32 x 128 pixels
+
+// ------------------------------------------------------------------------
+// ICIA's error moment (TU/cu/ICIA.h:346-369 with ICIAErrorMoment, :61-112):
+// diffH and diffV of the image from one read (the fused pass, mask 0x06),
+// and, without writing them, the sum over every pixel of the upper triangle
+// of a a^T, a = image_derivative0(s u, s v, eH, eV) of the affine (D = 6,
+// TU/cu/vec.h:1418-1422) or projective (D = 8, :1285-1290) map, s = 1 /
+// max(nrow, ncol), u, v the pixel's column and row.  A gradient is 0 outside
+// its valid region, as the reference's freshly allocated gradient images
+// are there.  Each workgroup sums its tile (FP32 per lane, FP64 across lanes
+// and waves) into partial[blockIdx.x]; icia_reduce_kernel adds the partials
+// in a fixed order.
+
+// Sum over the wave, the result in every lane.
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+template <int D> constexpr int icia_nm() { return D * (D + 1) / 2; }
+
+template <int RC, class IN, int D>
+__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(MFMA_WAVES_PER_SIMD<IN>)))
+icia_mfma_kernel(const SepArgs a, double* __restrict__ partial)
+{
+    constexpr int WY = MFMA_WYS[MFMA_WY_DEFAULT];
+    constexpr int NM = icia_nm<D>();
+    __shared__ MfmaTabs T;
+    __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    __shared__ float red[MWAVES][NM];
+    int y0, x0;
+    const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
+    MfmaRaw raw;
+    if (live) mfma_load<RC, IN>(a, y0, x0, raw);
+    mfma_build_tabs<MODE_GAUSS>(a, T);
+    __syncthreads();
+    float mom[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) mom[k] = 0.f;
+    if (live) {
+        v4f32 gH[MNB], gV[MNB];
+        mfma_compute<RC, IN, float, MODE_ICIA>(
+            a, y0, x0, raw, T, reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH, [] {},
+            [&](int k, const v4f32* acc, float u) {
+#pragma unroll
+                for (int o = 0; o < MNB; ++o) {
+                    if (k == 1) gH[o] = acc[o] * u;
+                    else        gV[o] = acc[o] * u;
+                }
+            });
+        // lane (g, m): row y0 + m, columns x0 + 16 o + 4 g + i of the accumulators
+        const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+        const float sc = 1.f / float(max(a.nrow, a.ncol));
+        const int y = y0 + m, r0 = a.r[0], r1 = a.r[1];
+        const bool rowH = y >= r0 && y < a.nrow - r0;          // diffH: V lobe 0, H lobe 1
+        const bool rowV = y >= r1 && y < a.nrow - r1;          // diffV: V lobe 1, H lobe 0
+        const float vf = sc * float(y);
+#pragma unroll
+        for (int o = 0; o < MNB; ++o)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + 16 * o + 4 * g + i;
+                const float eH = rowH && x >= r1 && x < a.ncol - r1 ? gH[o][i] : 0.f;
+                const float eV = rowV && x >= r0 && x < a.ncol - r0 ? gV[o][i] : 0.f;
+                const float uf = sc * float(x);
+                float av[D];
+                av[0] = eH * uf; av[1] = eH * vf; av[2] = eH;
+                av[3] = eV * uf; av[4] = eV * vf; av[5] = eV;
+                if constexpr (D == 8) {
+                    const float t = -(eH * uf + eV * vf);
+                    av[6] = t * uf;
+                    av[7] = t * vf;
+                }
+                int k = 0;
+#pragma unroll
+                for (int p = 0; p < D; ++p)
+#pragma unroll
+                    for (int q = p; q < D; ++q, ++k) mom[k] = __builtin_fmaf(av[p], av[q], mom[k]);
+            }
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        const float t = wave_sum(mom[k]);
+        if ((threadIdx.x & 63) == 0) red[w][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < NM) {
+        double t = 0.0;
+#pragma unroll
+        for (int v = 0; v < MWAVES; ++v) t += double(red[v][threadIdx.x]);
+        partial[int64_t(blockIdx.x) * NM + threadIdx.x] = t;
+    }
+}
+
+// moment[k] = the sum of partial[b][k] over the nblocks workgroups, in FP64,
+// in a fixed order (a strided sum per thread, then a tree); 0 when nblocks is 0.
+template <int NM>
+__global__ void __launch_bounds__(256) icia_reduce_kernel(const double* __restrict__ partial, int nblocks,
+                                                          float* __restrict__ moment)
+{
+    __shared__ double sh[256];
+    for (int k = 0; k < NM; ++k) {
+        double t = 0.0;
+        for (int b = int(threadIdx.x); b < nblocks; b += 256) t += partial[int64_t(b) * NM + k];
+        sh[threadIdx.x] = t;
+        __syncthreads();
+        for (int d = 128; d > 0; d >>= 1) {
+            if (int(threadIdx.x) < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) moment[k] = float(sh[0]);
+        __syncthreads();
+    }
+}
+
+template <int RC, class IN, int D>
+hipError_t launch_icia(const SepArgs& a, int nblocks, double* partial, float* moment, hipStream_t s)
+{
+    if (nblocks > 0)
+        hipLaunchKernelGGL((icia_mfma_kernel<RC, IN, D>), dim3(nblocks), dim3(64 * MWAVES), 0, s, a, partial);
+    hipLaunchKernelGGL(icia_reduce_kernel<icia_nm<D>()>, dim3(1), dim3(256), 0, s, partial, nblocks, moment);
+    return hipGetLastError();
+}
+
+using IciaLaunchFn = hipError_t (*)(const SepArgs&, int, double*, float*, hipStream_t);
+
 extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 extern KernelEntry g_fir_mfma_table[2][4][4];       // MODE_TAPS, default shape: [radius class][in][out]
+extern IciaLaunchFn g_icia_table[2][4][2];          // [radius class][in][affine, projective]
 
 }   // namespace tucu

@@ -36,7 +36,9 @@
 
 namespace tucu {
 
-enum { MODE_FIR = 0, MODE_GAUSS = 1, MODE_TAPS = 2 };
+// MODE_ICIA: the fused diffH / diffV pass of ICIA's error moment; the two
+// gradients go to the caller's sink instead of memory (mfma.h).
+enum { MODE_FIR = 0, MODE_GAUSS = 1, MODE_TAPS = 2, MODE_ICIA = 3 };
 constexpr int TMAX = 33;      // full taps, radius <= 16
 constexpr int NT = 256;        // threads per workgroup = input columns per tile
 constexpr int KMAX = 17;       // centre-indexed coefficients, radius <= 16
@@ -264,9 +266,13 @@ vpass_sym(const A& a, const float* w, float* lds, int t)
             if (F2) acc2[i] = k == RV ? a.cv[2][k] * s : __builtin_fmaf(a.cv[2][k], s, acc2[i]);
         }
     }
+    // An antisymmetric FIR lobe (sgnv < 0) has no centre tap: its centre
+    // pixel is not multiplied at all (a zero weight would turn an Inf there
+    // into NaN), as FIRFilter.h:66-176 leaves it out.
+    const bool centre = !FIR || a.sgnv > 0.f;
 #pragma unroll
     for (int i = 0; i < TH; ++i) {
-        if (F0) lds[(0 * TH + i) * PITCH + t] = __builtin_fmaf(a.cv[0][0], w[i + RV], acc0[i]);
+        if (F0) lds[(0 * TH + i) * PITCH + t] = centre ? __builtin_fmaf(a.cv[0][0], w[i + RV], acc0[i]) : acc0[i];
         if (F2) lds[(2 * TH + i) * PITCH + t] = __builtin_fmaf(a.cv[2][0], w[i + RV], acc2[i]);
     }
 }
@@ -374,9 +380,10 @@ hpass_field(const A& a, const float* lds, int t, int cy, int cx)
             // them costs 64 registers and occupancy.
             __builtin_amdgcn_sched_barrier(0);
         }
+        const bool centre = !FIR || a.sgnh > 0.f;       // no centre tap for an antisymmetric FIR lobe
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            r0[q] = __builtin_fmaf(c0[0], v[q + RH], r0[q]);
+            r0[q] = centre ? __builtin_fmaf(c0[0], v[q + RH], r0[q]) : r0[q];
             if (H2) r2[q] = __builtin_fmaf(a.cv[2][0], v[q + RH], r2[q]);
         }
         }

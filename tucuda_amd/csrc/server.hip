@@ -2,7 +2,9 @@
 //
 // Requests (an input image already in device memory, sigma, the outputs
 // wanted) are submitted from any thread.  A scheduler thread admits them in
-// arrival order while the output pool (a fixed HBM budget) has room, packs
+// arrival order while the output pool has room -- the pool grows in chunks
+// while the device's free HBM (hipMemGetInfo) less a reserve allows, up to
+// an optional cap (tucu_server_admit_bytes is the rule) -- packs
 // every admitted request into the batched kernel's job and tile tables, and
 // launches it on one of a few in-flight slots (a stream each), while earlier
 // batches are still running; a request that arrives while the GPU is busy
@@ -43,49 +45,72 @@ constexpr size_t ALIGN = 256;
 
 size_t round_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
-// First-fit allocator of offsets in one device arena, with coalescing.
+// First-fit allocator over device chunks (one arena at create, more as the
+// admission rule lets the pool grow), with coalescing inside each chunk.
 class Pool {
   public:
-    void init(size_t cap) { cap_ = cap; free_.clear(); free_[0] = cap; used_ = 0; peak_ = 0; }
-    bool alloc(size_t n, size_t& off)
+    struct Chunk {
+        char*                    base = nullptr;
+        size_t                   cap = 0;
+        std::map<size_t, size_t> free_;           // offset -> length
+    };
+
+    void add(char* base, size_t cap)
+    {
+        Chunk c;
+        c.base = base;
+        c.cap = cap;
+        c.free_[0] = cap;
+        chunks_.push_back(std::move(c));
+        cap_ += cap;
+    }
+    bool alloc(size_t n, int& chunk, size_t& off)
     {
         n = round_up(n, ALIGN);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < n) continue;
-            off = it->first;
-            const size_t rest = it->second - n;
-            free_.erase(it);
-            if (rest) free_[off + n] = rest;
-            used_ += n;
-            peak_ = std::max(peak_, used_);
-            return true;
+        for (size_t ci = 0; ci < chunks_.size(); ++ci) {
+            auto& fr = chunks_[ci].free_;
+            for (auto it = fr.begin(); it != fr.end(); ++it) {
+                if (it->second < n) continue;
+                off = it->first;
+                const size_t rest = it->second - n;
+                fr.erase(it);
+                if (rest) fr[off + n] = rest;
+                used_ += n;
+                peak_ = std::max(peak_, used_);
+                chunk = int(ci);
+                return true;
+            }
         }
         return false;
     }
-    void release(size_t off, size_t n)
+    void release(int chunk, size_t off, size_t n)
     {
         n = round_up(n, ALIGN);
         used_ -= n;
-        auto it = free_.emplace(off, n).first;
+        auto& fr = chunks_[size_t(chunk)].free_;
+        auto it = fr.emplace(off, n).first;
         auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
+        if (nx != fr.end() && it->first + it->second == nx->first) {
             it->second += nx->second;
-            free_.erase(nx);
+            fr.erase(nx);
         }
-        if (it != free_.begin()) {
+        if (it != fr.begin()) {
             auto pv = std::prev(it);
             if (pv->first + pv->second == it->first) {
                 pv->second += it->second;
-                free_.erase(it);
+                fr.erase(it);
             }
         }
     }
+    char*  base(int chunk) const { return chunks_[size_t(chunk)].base; }
     size_t cap()  const { return cap_; }
     size_t used() const { return used_; }
     size_t peak() const { return peak_; }
+    size_t nchunks() const { return chunks_.size(); }
+    const std::vector<Chunk>& chunks() const { return chunks_; }
 
   private:
-    std::map<size_t, size_t> free_;
+    std::vector<Chunk> chunks_;
     size_t cap_ = 0, used_ = 0, peak_ = 0;
 };
 
@@ -93,6 +118,7 @@ struct Request {
     uint64_t     id = 0;
     tucu_request req;
     size_t       off[6] = {0}, bytes[6] = {0};
+    int          chunk[6] = {0};
     tucu_image   out[6];
     double       admitted = 0, launched = 0;
     bool         done = false;     // completed (guarded by the server's mu); until then the
@@ -121,7 +147,6 @@ struct tucu_server_s {
     int                                    thi = 0;      // tile-height index of the batched kernel
     int                                    device = 0;
     std::chrono::steady_clock::time_point  t0;
-    char*                                  arena = nullptr;
     std::mutex                             pool_mu;
     Pool                                   pool;
     std::vector<Slot>                      slots;
@@ -167,7 +192,7 @@ void release_outputs(tucu_server_s& s, Request* r)
 {
     std::lock_guard<std::mutex> lock(s.pool_mu);
     for (int k = 0; k < 6; ++k)
-        if (r->bytes[k]) { s.pool.release(r->off[k], r->bytes[k]); r->bytes[k] = 0; }
+        if (r->bytes[k]) { s.pool.release(r->chunk[k], r->off[k], r->bytes[k]); r->bytes[k] = 0; }
 }
 
 void complete(tucu_server_s& s, Request* r, int status, double t)
@@ -192,27 +217,65 @@ void complete(tucu_server_s& s, Request* r, int status, double t)
     s.done_reqs++;
 }
 
+// Bytes of pool memory the outputs of request r take.
+size_t output_bytes(const tucu_server_s& s, const tucu_request& q)
+{
+    const size_t es = dtype_size(s.cfg.out_dtype);
+    const size_t row = round_up(size_t(q.in.ncol) * es, 128);
+    return size_t(__builtin_popcount(q.mask)) * round_up(row * size_t(q.in.nrow), ALIGN);
+}
+
 // Output memory of request r from the pool; false if the pool is full.
-bool allocate_outputs(tucu_server_s& s, Request* r)
+bool try_allocate(tucu_server_s& s, Request* r)
 {
     const int es = int(dtype_size(s.cfg.out_dtype));
     const int32_t h = r->req.in.nrow, w = r->req.in.ncol;
     const int64_t stride = int64_t(round_up(size_t(w) * es, 128) / es);
-    std::lock_guard<std::mutex> lock(s.pool_mu);
     for (int k = 0; k < 6; ++k) {
         if (!(r->req.mask >> k & 1u)) continue;
         const size_t bytes = size_t(h) * stride * es;
         size_t off;
-        if (!s.pool.alloc(bytes, off)) {
+        int ch;
+        if (!s.pool.alloc(bytes, ch, off)) {
             for (int j = 0; j < k; ++j)
-                if (r->bytes[j]) { s.pool.release(r->off[j], r->bytes[j]); r->bytes[j] = 0; }
+                if (r->bytes[j]) { s.pool.release(r->chunk[j], r->off[j], r->bytes[j]); r->bytes[j] = 0; }
             return false;
         }
         r->off[k] = off;
+        r->chunk[k] = ch;
         r->bytes[k] = bytes;
-        r->out[k] = {s.arena + off, stride, h, w, s.cfg.out_dtype, 0};
+        r->out[k] = {s.pool.base(ch) + off, stride, h, w, s.cfg.out_dtype, 0};
     }
     return true;
+}
+
+// Output memory of request r: from the pool, or from a new chunk when the
+// admission rule (tucu_server_admit_bytes: the device's free memory less the
+// reserve, within the pool cap) allows one.  1: allocated; 0: wait for
+// memory to be released; -1: the request can never fit (nothing is in use,
+// and the device cannot hold it).
+int allocate_outputs(tucu_server_s& s, Request* r)
+{
+    std::lock_guard<std::mutex> lock(s.pool_mu);
+    if (try_allocate(s, r)) return 1;
+    size_t dev_free = 0, dev_total = 0;
+    if (hipMemGetInfo(&dev_free, &dev_total) != hipSuccess) {
+        (void)hipGetLastError();
+        dev_free = 0;
+    }
+    const size_t need = output_bytes(s, r->req) + ALIGN;
+    const size_t grow = tucu_server_admit_bytes(need, dev_free, s.cfg.reserve_bytes, s.pool.cap(),
+                                                s.cfg.max_pool_bytes, s.cfg.grow_bytes);
+    if (grow > 0) {
+        void* p = nullptr;
+        if (hipMalloc(&p, grow) == hipSuccess && p) {
+            s.pool.add(static_cast<char*>(p), grow);
+            if (try_allocate(s, r)) return 1;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    return s.pool.used() == 0 ? -1 : 0;
 }
 
 // Reaps finished slots; returns true if any finished.
@@ -257,8 +320,15 @@ bool admit_and_launch(tucu_server_s& s, Slot& sl)
         // completions (and the pool memory they free) come steadily
         const int64_t px = int64_t(r->req.in.nrow) * r->req.in.ncol;
         if (!jobs.empty() && pixels + px > max_pixels) break;
+        const int got = allocate_outputs(s, r);              // admission by free HBM: FIFO, head blocks
+        if (got == 0) break;
+        if (got < 0) {                                        // can never fit: fail it, go on
+            s.waiting.pop_front();
+            r->admitted = r->launched = s.now();
+            complete(s, r, TUCU_ERR_BAD_ALLOC, s.now());
+            continue;
+        }
         pixels += px;
-        if (!allocate_outputs(s, r)) break;                   // admission by free HBM: FIFO, head blocks
         const Lobes& l = lobes_for(s, r->req.sigma);
         const float* lob[3] = {&l[0], &l[17], &l[34]};
         const int n[3] = {int(l[51]), int(l[52]), int(l[53])};
@@ -340,6 +410,11 @@ void scheduler(tucu_server_s* s)
     }
 }
 
+void free_pool(tucu_server_s* s)
+{
+    for (const auto& c : s->pool.chunks()) (void)hipFree(c.base);
+}
+
 void destroy_slots(tucu_server_s* s)
 {
     for (Slot& sl : s->slots) {
@@ -358,6 +433,16 @@ void destroy_slots(tucu_server_s* s)
 
 extern "C" {
 
+size_t tucu_server_admit_bytes(size_t need, size_t device_free, size_t reserve, size_t pool_bytes,
+                               size_t max_pool_bytes, size_t grow_step)
+{
+    const size_t avail = device_free > reserve ? device_free - reserve : 0;
+    const size_t room = max_pool_bytes == 0 ? SIZE_MAX : max_pool_bytes > pool_bytes ? max_pool_bytes - pool_bytes : 0;
+    const size_t limit = std::min(avail, room);
+    if (need == 0 || need > limit) return 0;
+    return std::min(limit, std::max(need, grow_step));
+}
+
 int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
 {
     if (!server || !cfg) return fail(TUCU_ERR_INVALID, "null argument");
@@ -373,9 +458,13 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
             batch_entry(fam, i, cfg->in_dtype, cfg->out_dtype).fn)
             thi = i;
     if (thi < 0) return fail(TUCU_ERR_INVALID, "no batched kernel for this tile height and type pair");
-    if (cfg->pool_bytes == 0) return fail(TUCU_ERR_INVALID, "empty output pool");
+    if (cfg->pool_bytes == 0 && cfg->max_pool_bytes != 0 && cfg->max_pool_bytes < ALIGN)
+        return fail(TUCU_ERR_INVALID, "empty output pool");
     auto s = new tucu_server_s();
     s->cfg = *cfg;
+    if (s->cfg.max_pool_bytes != 0) s->cfg.max_pool_bytes = std::max(s->cfg.max_pool_bytes, s->cfg.pool_bytes);
+    if (s->cfg.reserve_bytes == 0) s->cfg.reserve_bytes = size_t(1) << 30;
+    if (s->cfg.grow_bytes == 0) s->cfg.grow_bytes = size_t(256) << 20;
     s->fam = fam;
     s->thi = thi;
     if (s->cfg.max_batch_jobs <= 0) s->cfg.max_batch_jobs = 1024;
@@ -384,12 +473,15 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
     if (s->cfg.max_batch_mpix <= 0) s->cfg.max_batch_mpix = 64;
     s->t0 = std::chrono::steady_clock::now();
     (void)hipGetDevice(&s->device);
-    if (hipMalloc(&s->arena, s->cfg.pool_bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        delete s;
-        return fail(TUCU_ERR_BAD_ALLOC, "server output pool allocation failed");
+    if (s->cfg.pool_bytes > 0) {
+        void* arena = nullptr;
+        if (hipMalloc(&arena, s->cfg.pool_bytes) != hipSuccess || !arena) {
+            (void)hipGetLastError();
+            delete s;
+            return fail(TUCU_ERR_BAD_ALLOC, "server output pool allocation failed");
+        }
+        s->pool.add(static_cast<char*>(arena), s->cfg.pool_bytes);
     }
-    s->pool.init(s->cfg.pool_bytes);
     s->slots.resize(size_t(s->cfg.max_inflight));
     for (Slot& sl : s->slots) {
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);   // default priority
@@ -401,7 +493,7 @@ int tucu_server_create(tucu_server* server, const tucu_server_config* cfg)
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_tiles, size_t(s->cfg.max_tiles) * sizeof(BatchTile));
         if (e != hipSuccess) {
             destroy_slots(s);
-            (void)hipFree(s->arena);
+            free_pool(s);
             delete s;
             return device_fail(e, "server slot setup");
         }
@@ -419,7 +511,7 @@ int tucu_server_destroy(tucu_server server)
     if (server->thread.joinable()) server->thread.join();
     destroy_slots(server);
     for (auto& kv : server->live) delete kv.second;
-    (void)hipFree(server->arena);
+    free_pool(server);
     delete server;
     return TUCU_OK;
 }
@@ -438,10 +530,9 @@ int check_request(const tucu_server_s& s, const tucu_request& req)
     if (!(req.sigma > 0.f) || !(req.sigma < 1e6f))                 // the rule tucu_gauss2_lobes applies
         return fail(TUCU_ERR_INVALID, "sigma must be positive and finite");
     if (!req.in.data || req.in.nrow <= 0 || req.in.ncol <= 0) return fail(TUCU_ERR_INVALID, "bad input image");
-    const size_t es = dtype_size(s.cfg.out_dtype);
-    const size_t row = round_up(size_t(req.in.ncol) * es, 128);
-    if (size_t(__builtin_popcount(req.mask)) * round_up(row * size_t(req.in.nrow), ALIGN) > s.cfg.pool_bytes)
-        return fail(TUCU_ERR_BAD_ALLOC, "request larger than the server's output pool");
+    const size_t cap = s.cfg.max_pool_bytes != 0 ? s.cfg.max_pool_bytes : SIZE_MAX;
+    if (output_bytes(s, req) > cap)
+        return fail(TUCU_ERR_BAD_ALLOC, "request larger than the server's output pool can grow");
     return TUCU_OK;
 }
 

@@ -278,6 +278,34 @@ class FIRGaussianConvolver2(FIRFilter2):
             ctypes.byref(src), descs, mask, int(bool(shift)), _stream_handle(inp)))
 
 
+ICIA_MODELS = {"affine": _capi.TUCU_ICIA_AFFINE, "projective": _capi.TUCU_ICIA_PROJECTIVE}
+
+
+def icia_moment(sigma_or_lobes, inp: torch.Tensor, model: str = "affine",
+                rule: int = _capi.TUCU_RULE_REFERENCE) -> torch.Tensor:
+    """ICIA's error moment (ICIA.h:346-369): diffH and diffV of ``inp`` from
+    one fused pass that never writes them, and the sum over every pixel of
+    the upper triangle of a a^T, a = image_derivative0(s u, s v, eH, eV) of
+    the affine (21 moments) or projective (36) map.  Returns a float32
+    device tensor of the moments, enqueued on torch's current stream."""
+    if isinstance(sigma_or_lobes, (int, float)):
+        l0, l1, _ = gauss_lobes(float(sigma_or_lobes), rule)
+    else:
+        l0, l1 = _f32(sigma_or_lobes[0]), _f32(sigma_or_lobes[1])
+    m = ICIA_MODELS[model]
+    src = image_desc(inp)
+    need = ctypes.c_size_t(0)
+    lib = _capi.load()
+    check(lib.tucu_icia_moment(_fptr(l0), l0.size, _fptr(l1), l1.size, ctypes.byref(src), m, None, None,
+                               ctypes.byref(need), _stream_handle(inp)))
+    ws = torch.empty(int(need.value), dtype=torch.uint8, device=inp.device)
+    out = torch.empty(m * (m + 1) // 2, dtype=torch.float32, device=inp.device)
+    check(lib.tucu_icia_moment(_fptr(l0), l0.size, _fptr(l1), l1.size, ctypes.byref(src), m,
+                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()),
+                               ctypes.byref(need), _stream_handle(inp)))
+    return out
+
+
 def batch_tile_rows():
     """Tile heights the batched kernels offer: the matrix-core kernels
     (default) take 16, 32 or 64 rows (workgroups of 4 x 1, 2 x 2 or 1 x 4

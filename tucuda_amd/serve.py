@@ -24,10 +24,15 @@ from .filters import _DTYPES, image_desc
 class Server:
     def __init__(self, pool_bytes: int, in_dtype=torch.bfloat16, out_dtype=torch.bfloat16,
                  rule: int = _capi.TUCU_RULE_CEIL4SIGMA, tile_rows: int = 0, max_batch_jobs: int = 1024,
-                 max_inflight: int = 2, max_tiles: int = 0, max_batch_mpix: int = 0):
+                 max_inflight: int = 2, max_tiles: int = 0, max_batch_mpix: int = 0,
+                 max_pool_bytes: int = 0, reserve_bytes: int = 0, grow_bytes: int = 0):
+        """``pool_bytes`` of output memory are reserved at once; the pool grows
+        while the device's free memory less ``reserve_bytes`` (0: 1 GiB)
+        allows, up to ``max_pool_bytes`` (0: no cap; pass pool_bytes for a
+        fixed pool)."""
         cfg = TucuServerConfig(int(pool_bytes), _DTYPES[in_dtype], _DTYPES[out_dtype], int(rule),
                                int(tile_rows), int(max_batch_jobs), int(max_inflight), int(max_tiles),
-                               int(max_batch_mpix))
+                               int(max_batch_mpix), 0, int(max_pool_bytes), int(reserve_bytes), int(grow_bytes))
         h = ctypes.c_void_p()
         check(_capi.load().tucu_server_create(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h
