@@ -263,6 +263,21 @@ TUCU_API int tucu_batch_set_tile_rows(tucu_batch batch, int rows);
 TUCU_API int tucu_batch_autotune(tucu_batch batch, void* stream, int reps, int* best_rows, float* best_us);
 TUCU_API int tucu_batch_info(tucu_batch batch, int* njobs, int* ntiles, int* grid, int* tile_rows);
 
+/* One batched launch of `jobs` that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Validates the jobs as tucu_batch_create does, builds the job table and
+ * tile list in host memory, copies them (hipMemcpyAsync from pageable
+ * memory, which returns once the host tables are consumed) into a static
+ * device scratch array of TUCU_STATIC_SCRATCH_BYTES that each GPU holds
+ * until the process ends, and launches the batched kernel there, with the
+ * default tile shape.  Results equal tucu_batch_launch's bit for bit.  A
+ * batch whose tables exceed the scratch is refused with TUCU_ERR_INVALID.
+ * Every call on one GPU shares that GPU's scratch: calls ordered on one
+ * stream are safe, calls that overlap on one GPU (other streams, other
+ * threads) race. */
+#define TUCU_STATIC_SCRATCH_BYTES (4u << 20)
+TUCU_API int tucu_batch_run_static(const tucu_gauss_job* jobs, int njobs, void* stream);
 /* Tuning cache as JSON lines. */
 TUCU_API int tucu_tune_cache_save(const char* path);
 TUCU_API int tucu_tune_cache_load(const char* path);

@@ -35,7 +35,7 @@ EXPORTS = (
     "tucu_lobe_size", "tucu_subsample", "tucu_gauss2_pyramid", "tucu_gauss2_lobes", "tucu_gauss2_convolve",
     "tucu_icia_moment",
     "tucu_batch_create", "tucu_batch_destroy", "tucu_batch_launch", "tucu_batch_set_tile_rows",
-    "tucu_batch_autotune", "tucu_batch_info", "tucu_tune_cache_save",
+    "tucu_batch_autotune", "tucu_batch_info", "tucu_batch_run_static", "tucu_tune_cache_save",
     "tucu_tune_cache_load",
     "tucu_server_admit_bytes", "tucu_server_create", "tucu_server_destroy", "tucu_server_now", "tucu_server_submit",
     "tucu_server_submit_many", "tucu_server_poll", "tucu_server_release", "tucu_server_stats",
@@ -191,6 +191,7 @@ def load() -> ctypes.CDLL:
     L.tucu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TucuGaussJob), ctypes.c_int]
     L.tucu_batch_destroy.argtypes = [vp]
     L.tucu_batch_launch.argtypes = [vp, vp]
+    L.tucu_batch_run_static.argtypes = [ctypes.POINTER(TucuGaussJob), ctypes.c_int, vp]
     L.tucu_batch_set_tile_rows.argtypes = [vp, ctypes.c_int]
     L.tucu_batch_autotune.argtypes = [vp, vp, ctypes.c_int, ip, fp]
     L.tucu_batch_info.argtypes = [vp, ip, ip, ip, ip]

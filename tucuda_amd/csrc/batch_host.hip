@@ -199,9 +199,79 @@ int batch_jobs(const tucu_gauss_job* jobs, int njobs, std::vector<SepArgs>& out,
     return TUCU_OK;
 }
 
+// ------------------------------------------------------ no-allocation launch
+// Scratch of tucu_batch_run_static: the tile-queue counters, then the job
+// table, then the tile list of one launch.  A file-scope device array: each
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never passes the array's name to a kernel or a copy (on the host it
+// is only a stand-in for the symbol); it uses the device address
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr size_t SCRATCH_BYTES = TUCU_STATIC_SCRATCH_BYTES;
+constexpr size_t SCRATCH_JOBS_OFF = 1024;                   // after the counters (QUEUE_BYTES)
+static_assert(QUEUE_BYTES <= SCRATCH_JOBS_OFF, "counters overlap the job table");
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void*      g_scratch_ptr[MAX_DEVICES] = {nullptr};
+
+// Device address of the current device's scratch.
+int scratch_ptr(unsigned char** p)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return device_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(TUCU_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return device_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *p = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return TUCU_OK;
+}
+
+size_t round256(size_t n) { return (n + 255) / 256 * 256; }
+
 }   // namespace
 
 extern "C" {
+
+int tucu_batch_run_static(const tucu_gauss_job* jobs, int njobs, void* stream)
+{
+    std::vector<SepArgs> args;
+    int in_dt = 0, out_dt = 0, rclass = 0;
+    int64_t work = 0;
+    int rc = batch_jobs(jobs, njobs, args, in_dt, out_dt, rclass, work);
+    if (rc || args.empty()) return rc;
+    const BatchEntry& e = batch_entry(batch_family(), BATCH_TH_DEFAULT, in_dt, out_dt);
+    if (!e.fn) return fail(TUCU_ERR_INVALID, "no batched kernel for this type pair");
+    std::vector<BatchTile> tiles;
+    if ((rc = make_tiles(args, e, tiles))) return rc;
+    if (tiles.empty()) return TUCU_OK;
+    const size_t jobs_bytes = round256(args.size() * sizeof(SepArgs));
+    const size_t tiles_off = SCRATCH_JOBS_OFF + jobs_bytes;
+    const size_t need = tiles_off + tiles.size() * sizeof(BatchTile);
+    if (need > SCRATCH_BYTES)
+        return fail(TUCU_ERR_INVALID, "batch tables need " + std::to_string(need) + " bytes; the static scratch holds " +
+                                      std::to_string(SCRATCH_BYTES));
+    unsigned char* base = nullptr;
+    if ((rc = scratch_ptr(&base))) return rc;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    SepArgs* d_jobs = reinterpret_cast<SepArgs*>(base + SCRATCH_JOBS_OFF);
+    BatchTile* d_tiles = reinterpret_cast<BatchTile*>(base + tiles_off);
+    // Pageable host memory: each copy returns once its source is consumed,
+    // in stream order behind the work already on `s`.
+    hipError_t err = hipMemcpyAsync(d_jobs, args.data(), args.size() * sizeof(SepArgs), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice, s);
+    if (err != hipSuccess) return device_fail(err, "hipMemcpyAsync (static scratch)");
+    // The counters are set by the launch's own reset kernel.
+    err = e.fn(d_jobs, d_tiles, int(tiles.size()), reinterpret_cast<int*>(base), batch_grid(e, int(tiles.size())), s);
+    return err == hipSuccess ? TUCU_OK : device_fail(err, "batched kernel launch");
+}
 
 int tucu_batch_create(tucu_batch* batch, const tucu_gauss_job* jobs, int njobs)
 {
