@@ -109,7 +109,8 @@ def _membench():
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built (make -C tucuda_amd/csrc)")
     L = ctypes.CDLL(path)
-    L.tucu_membench_fill.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_void_p]
+    L.tucu_membench_fill.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p]
     L.tucu_membench_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     return L
 
@@ -146,7 +147,7 @@ def write_ceiling(dev, out_bytes: int) -> dict:
     with torch.cuda.stream(st):
         with torch.cuda.graph(graph, stream=st):
             for b in bufs:
-                assert L.tucu_membench_fill(ctypes.c_void_p(b.data_ptr()), nb, 1.0,
+                assert L.tucu_membench_fill(ctypes.c_void_p(b.data_ptr()), nb, 1.0, 1, 0,
                                             ctypes.c_void_p(st.cuda_stream)) == 0
     stream.wait_stream(st)
     t_small = events(graph.replay, 20) / len(bufs)
@@ -154,7 +155,7 @@ def write_ceiling(dev, out_bytes: int) -> dict:
     big = 1 << 30
     a = torch.empty(big // 4, dtype=torch.float32, device=dev)
     b = torch.empty(big // 4, dtype=torch.float32, device=dev)
-    t_fill = events(lambda: L.tucu_membench_fill(ctypes.c_void_p(a.data_ptr()), big, 2.0, sh), 10)
+    t_fill = events(lambda: L.tucu_membench_fill(ctypes.c_void_p(a.data_ptr()), big, 2.0, 1, 0, sh), 10)
     t_copy = events(lambda: L.tucu_membench_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
                                                  big, sh), 10)
     del a, b
