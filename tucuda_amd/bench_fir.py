@@ -1,18 +1,21 @@
-"""FIRFilter2 / general-taps measurement (one output per launch).
+"""FIRFilter2 / general-taps and 16-bit storage measurement.
 
-    python -m tucuda_amd.bench_fir
+    python -m tucuda_amd.bench_fir [--cases a,b] [--families mfma,valu] [--reps 20]
 
 Workloads: the 4096x4096 u8 image filtered by 17 x 9 general taps (N(0,1),
 unit L1 norm, seeded) into float32 and into float16, and by FIRFilter2 lobes
-of 9 / 5 (17 x 9 symmetric taps); and a 1920x1080 float32 image by 17 / 17
-lobes.  Each is timed on both kernel families (matrix cores, the default,
-and the direct FP32 vector kernel, TUCU_FIR_PATH=valu), launches replayed
-from a captured HIP graph over 4 rotating buffer sets.  Prints one JSON line
-with microseconds per launch and the algorithmic bytes rate (input read
-once, the valid output region written once).
+of 9 / 5 (17 x 9 symmetric taps); a 1920x1080 float32 image by 17 / 17
+lobes; and the six-output Gaussian (sigma 3) of a 1920x1080 image stored in
+float32, bfloat16 and float16, into outputs of the same type.  Each is timed on
+both kernel families (matrix cores, the default, and the direct FP32 vector
+kernel, TUCU_FIR_PATH / TUCU_GAUSS_PATH=valu), launches replayed from a
+captured HIP graph over 4 rotating buffer sets.  Prints one JSON line with
+microseconds per launch and the algorithmic bytes rate (input read once,
+the valid output region of each output written once).
 """
 from __future__ import annotations
 
+import argparse
 import json
 import os
 import sys
@@ -21,6 +24,11 @@ import numpy as np
 
 
 def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cases", default="", help="comma-separated workload names (default: all)")
+    ap.add_argument("--families", default="mfma,valu")
+    ap.add_argument("--reps", type=int, default=20, help="graph replays timed per workload")
+    args = ap.parse_args(argv)
     import torch
     import tucuda_amd
     from . import synth
@@ -43,26 +51,42 @@ def main(argv=None):
     img1080 = torch.from_numpy(synth.edge_image(3, 1080, 1920)).to(dev)
     l17 = np.abs(unit(17)).astype(np.float32)
     cases.append(("lobes17x17_1080p_f32", img1080, torch.float32, ("lobes", l17, l17), 16, 16))
+    for dt in (torch.float32, torch.bfloat16, torch.float16):
+        cases.append(("gauss6_1080p_" + str(dt).split(".")[-1], img1080.to(dt), dt, ("gauss6", 3.0, None), 16, 16))
+    if args.cases:
+        want = set(args.cases.split(","))
+        unknown = want - {c[0] for c in cases}
+        if unknown:
+            raise SystemExit(f"unknown cases {sorted(unknown)}")
+        cases = [c for c in cases if c[0] in want]
 
     res = {"workloads": {}}
     for name, img, odt, (kind, h, v), rh, rv in cases:
-        outs = [torch.zeros(img.shape, dtype=odt, device=dev) for _ in range(4)]
+        nout = 6 if kind == "gauss6" else 1
+        outs = [[torch.zeros(img.shape, dtype=odt, device=dev) for _ in range(nout)] for _ in range(4)]
         ins = [img.clone() for _ in range(4)]
-        nbytes = img.numel() * img.element_size() + (img.shape[0] - 2 * rv) * (img.shape[1] - 2 * rh) * outs[0].element_size()
+        nbytes = img.numel() * img.element_size() + \
+            nout * (img.shape[0] - 2 * rv) * (img.shape[1] - 2 * rh) * outs[0][0].element_size()
         entry = {}
-        for fam in ("mfma", "valu"):
-            if fam == "valu":
-                os.environ["TUCU_FIR_PATH"] = "valu"
-            else:
-                os.environ.pop("TUCU_FIR_PATH", None)
+        for fam in args.families.split(","):
+            for var in ("TUCU_FIR_PATH", "TUCU_GAUSS_PATH"):
+                if fam == "valu":
+                    os.environ[var] = "valu"
+                else:
+                    os.environ.pop(var, None)
             if kind == "taps":
                 f = tucuda_amd.SeparableFilter2(h, v)
+            elif kind == "gauss6":
+                f = tucuda_amd.FIRGaussianConvolver2(h)
             else:
                 f = tucuda_amd.FIRFilter2().initialize(h, v)
 
             def run():
                 for x, o in zip(ins, outs):
-                    f.convolve(x, o)
+                    if kind == "gauss6":
+                        f.convolve6(x, o)
+                    else:
+                        f.convolve(x, o[0])
 
             run()
             torch.cuda.synchronize()
@@ -78,14 +102,16 @@ def main(argv=None):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             rs = torch.cuda.current_stream(dev)
             e0.record(rs)
-            for _ in range(20):
+            for _ in range(args.reps):
                 graph.replay()
             e1.record(rs)
             torch.cuda.synchronize()
-            us = e0.elapsed_time(e1) * 1000 / (20 * len(outs))
+            us = e0.elapsed_time(e1) * 1000 / (args.reps * len(outs))
             entry[fam] = {"us": round(us, 2), "gb_s": round(nbytes / us / 1e3, 1)}
             del graph
         os.environ.pop("TUCU_FIR_PATH", None)
+        os.environ.pop("TUCU_GAUSS_PATH", None)
+        print(f"{name}: {entry}", file=sys.stderr, flush=True)
         res["workloads"][name] = entry
     print(json.dumps(res), flush=True)
 

@@ -25,7 +25,8 @@
 // split residual are each below 2^-22 relative), summed in FP32.  Scales
 // (all powers of two): taps of lobe l by mp[l] (largest tap in [2^5, 2^6),
 // MFMA_TAP_EXP); the input of each strip by mq times a factor putting its
-// largest magnitude in [2^12, 2^13), mq <= 1 / (L1 norm * mp) of every lobe,
+// largest magnitude in [2^12, 2^13) (u8 input: the factor for 255, 2^5,
+// whatever the strip holds), mq <= 1 / (L1 norm * mp) of every lobe,
 // so that the vertical fields stay below 2^13 and no operand leaves the f16
 // range; each output is multiplied back by mu[k] / (the strip's factor).
 // Where lo falls below f16's normal range (2^-14) it keeps an absolute
@@ -172,21 +173,92 @@ __device__ __forceinline__ v8f16 cat8(v4f16 a, v4f16 b)
     return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
+// v of the lane that DPP control CTRL selects (within rows of 16 lanes).
+template <int CTRL> __device__ __forceinline__ float dpp_f32(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+
+// Largest v over the wave (every lane active), in every lane: DPP within
+// each row of 16 lanes (quad swaps, half-row and row mirrors), then the four
+// row results read as scalars.  (Six ds_bpermute rounds, each waiting for
+// the LDS, did the same with a longer dependent chain.)
 __device__ __forceinline__ float wave_max(float v)
 {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
-    return v;
+    v = fmaxf(v, dpp_f32<0xb1>(v));             // quad_perm [1, 0, 3, 2]
+    v = fmaxf(v, dpp_f32<0x4e>(v));             // quad_perm [2, 3, 0, 1]
+    v = fmaxf(v, dpp_f32<0x141>(v));            // row_half_mirror
+    v = fmaxf(v, dpp_f32<0x140>(v));            // row_mirror
+    const float a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+    const float b = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16));
+    const float c = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+    const float d = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 48));
+    return fmaxf(fmaxf(a, b), fmaxf(c, d));
+}
+
+// Two u8 values as f16 times s, exactly: with M = 1024 s a power of two in
+// f16's normal range, M's bits with x (< 256) in the low mantissa bits are
+// the f16 M + x s, and subtracting M leaves x s.  mb = M's bits in both
+// halves, nm = -M in both halves.
+__device__ __forceinline__ v2f16 u8pair_f16(uint32_t lo, uint32_t hi, uint32_t mb, v2f16 nm)
+{
+    const uint32_t w = (hi << 16) | lo | mb;
+    return __builtin_bit_cast(v2f16, w) + nm;
+}
+
+// One input value as loaded, unconverted: float, the 16 bits of an f16 /
+// bf16, or the byte.  mfma_compute converts them where it uses them, so that
+// no conversion sits right after the loads and makes the wave wait for them
+// there (the loads of a strip stay in flight through the table build, or
+// through the previous strip's work in the tile loop).
+// Narrow inputs are held zero-extended in 32 bits.
+template <class IN> using RawElem = std::conditional_t<sizeof(IN) == 4, float, uint32_t>;
+
+// Range-checked load of a float input (a pixel outside the image reads 0).
+template <class IN>
+__device__ __forceinline__ RawElem<IN> mfma_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    static_assert(sizeof(IN) == 4, "narrow inputs load with mfma_ld_in");
+    return Px<IN>::ld(r, voff, soff);
+}
+
+// Unchecked load at row base + lane offset, for windows inside the image:
+// a narrow value arrives zero-extended by the load itself (with the buffer
+// form the compiler masks it again, and that mask, placed right after the
+// loads, makes the wave wait for them there).
+template <class IN>
+__device__ __forceinline__ RawElem<IN> mfma_ld_in(const char* row, uint32_t off)
+{
+    const gptr<const uint8_t> base = reinterpret_cast<gptr<const uint8_t>>(reinterpret_cast<uintptr_t>(row));
+    if constexpr (sizeof(IN) == 4)
+        return *reinterpret_cast<const gptr<const float>>(base + off);
+    else if constexpr (sizeof(IN) == 2)
+        return *reinterpret_cast<const gptr<const uint16_t>>(base + off);
+    else
+        return base[off];
+}
+
+// The value of a loaded element as a float.
+template <class IN>
+__device__ __forceinline__ float raw_value(RawElem<IN> v)
+{
+    if constexpr (std::is_same_v<IN, f16_t>)
+        return float(__builtin_bit_cast(_Float16, uint16_t(v)));
+    else if constexpr (std::is_same_v<IN, bf16_t>)
+        return __uint_as_float(uint32_t(v) << 16);
+    else
+        return float(v);
 }
 
 // Input values of field block b for lane (g, m): column x0 - RC + 16 b + m,
 // rows y0 - RC + 8 g + j (j < 8) and, for K = 48, y0 - RC + 32 + 4 g + j
 // (j < 4).  INSIDE: every pixel of the strip's window lies in the image
 // (row offsets as scalar offsets); otherwise pixels outside the image read
-// as 0 (buffer range check).
+// as 0 (buffer range check for float input; for narrow input a load at the
+// clamped address and a select).
 template <int RC, bool INSIDE, class IN, class A>
 __device__ __forceinline__ void
-mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, int g, int m, float* v)
+mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, int g, int m, RawElem<IN>* v)
 {
     constexpr int NLD = MGeo<RC>::NLD;
     const int es = int(sizeof(IN));
@@ -194,13 +266,38 @@ mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, 
     const int sb = a.in_stride_b;
     const int r0 = y0 - RC + 8 * g;            // first row of entries 0 .. 7
     const int r1 = y0 - RC + 32 + 4 * g;       // first row of entries 8 .. 11
-    if constexpr (INSIDE) {
-        const int v0 = r0 * sb + col * es, v1 = r1 * sb + col * es;
+    if constexpr (INSIDE && sizeof(IN) < 4) {
+        // row bases uniform (y0 - RC + j), lane offsets 32-bit
+        const char* const in = static_cast<const char*>(a.in);
+        const uint32_t o0 = uint32_t(8 * g * sb + col * es), o1 = uint32_t((32 + 4 * g) * sb + col * es);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = Px<IN>::ld(rsrc, v0, j * sb);
+        for (int j = 0; j < 8; ++j) v[j] = mfma_ld_in<IN>(in + int64_t(y0 - RC + j) * sb, o0);
         if constexpr (MGeo<RC>::K48) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[8 + j] = Px<IN>::ld(rsrc, v1, j * sb);
+            for (int j = 0; j < 4; ++j) v[8 + j] = mfma_ld_in<IN>(in + int64_t(y0 - RC + j) * sb, o1);
+        }
+    } else if constexpr (INSIDE) {
+        const int v0 = r0 * sb + col * es, v1 = r1 * sb + col * es;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = mfma_ld<IN>(rsrc, v0, j * sb);
+        if constexpr (MGeo<RC>::K48) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[8 + j] = mfma_ld<IN>(rsrc, v1, j * sb);
+        }
+    } else if constexpr (sizeof(IN) < 4) {
+        // the same extending loads at addresses clamped into the image, and
+        // 0 where the pixel lies outside (so that the values of both paths
+        // meet without a mask after them)
+        const char* const in = static_cast<const char*>(a.in);
+        const bool colok = unsigned(col) < unsigned(a.ncol);
+        const int colc = min(max(col, 0), a.ncol - 1);
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int row = j < 8 ? r0 + j : r1 + (j - 8);
+            const bool ok = colok && unsigned(row) < unsigned(a.nrow);
+            const int rowc = min(max(row, 0), a.nrow - 1);
+            const RawElem<IN> x = mfma_ld_in<IN>(in + int64_t(rowc) * sb, uint32_t(colc * es));
+            v[j] = ok ? x : 0u;
         }
     } else {
         const bool colok = unsigned(col) < unsigned(a.ncol);
@@ -208,7 +305,7 @@ mfma_load_block(const A& a, __amdgpu_buffer_rsrc_t rsrc, int y0, int x0, int b, 
         for (int j = 0; j < NLD; ++j) {
             const int row = j < 8 ? r0 + j : r1 + (j - 8);
             const bool ok = colok && unsigned(row) < unsigned(a.nrow);
-            v[j] = Px<IN>::ld(rsrc, ok ? row * sb + col * es : MINVALID, 0);
+            v[j] = mfma_ld<IN>(rsrc, ok ? row * sb + col * es : MINVALID, 0);
         }
     }
 }
@@ -287,7 +384,7 @@ __device__ __forceinline__ float fp32_taps(const C& t, int r, const W& w)
 // nonzero) are c(k) on the upper / left side.
 template <int RC, class IN, class OUT, int MODE, class A>
 __device__ __forceinline__ void
-mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage)
+mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage, unsigned mask)
 {
     constexpr bool ONE = MODE == MODE_TAPS;
     constexpr int NFC = 32 + 2 * RC;            // field columns per half strip
@@ -296,10 +393,9 @@ mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage)
     // path is computed ahead of it (in a tile loop, such hoisted values would
     // hold registers across the matrix-core path).
     int lane = threadIdx.x & 63;
-    asm volatile("" : "+v"(lane), "+s"(y0), "+s"(x0));
+    asm volatile("" : "+v"(lane), "+s"(y0), "+s"(x0), "+s"(mask));
     const int es = int(sizeof(IN));
     const int sb = a.in_stride_b;
-    const unsigned mask = a.mask;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.in), short(0), int(a.in_bytes), BUF_DWORD3);
     constexpr int HL[6] = {ONE ? 1 : 0, 1, 0, 2, 1, 0};   // horizontal lobe of output k (TAPS: table 1)
@@ -388,7 +484,7 @@ mfma_strip_fp32(const A& a, int y0, int x0, const MfmaTabs& T, float* stage)
 //     by d = 32 p - 16 o (zero for the block outside the window).
 // A strip's input, as loaded: field block b, entry j (mfma_load_block's
 // order); radius class 8 uses blocks 0 .. 4 and entries 0 .. 7.
-typedef float MfmaRaw[6][12];
+template <class IN> using MfmaRaw = RawElem<IN>[6][12];
 
 // Register budget of the kernels: two waves per SIMD for float input (the
 // low halves of the input and of the tail rows stay live), three for 16-bit
@@ -409,7 +505,7 @@ template <int MODE, class IN> constexpr int MFMA_LOOP_WAVES = MODE == MODE_TAPS 
 
 // Issues the loads of the strip at (y0, x0); the values arrive in raw.
 template <int RC, class IN, class A>
-__device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& raw)
+__device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw<IN>& raw)
 {
     using G = MGeo<RC>;
     const int lane = threadIdx.x & 63;
@@ -430,7 +526,7 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
     }
     if constexpr (!G::K48) {
 #pragma unroll
-        for (int b = 0; b < G::NVB; ++b) raw[b][8] = raw[b][9] = raw[b][10] = raw[b][11] = 0.f;
+        for (int b = 0; b < G::NVB; ++b) raw[b][8] = raw[b][9] = raw[b][10] = raw[b][11] = 0;
     }
 }
 
@@ -455,8 +551,8 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw& r
 //     by d = 32 p - 16 o (zero for the block outside the window).
 template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F, class S>
 __device__ __forceinline__ void
-mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass,
-             S&& sink)
+mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, float* stage, F&& after_vpass,
+             S&& sink, unsigned mask)
 {
     constexpr bool ONE = MODE == MODE_TAPS;     // one field (table 0), one output (table 1 across)
     using G = MGeo<RC>;
@@ -466,22 +562,28 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     typedef _Float16 vfield __attribute__((ext_vector_type(NF)));
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, m = lane & 15;
-    const unsigned mask = a.mask;
     const v4f16 z4 = {0, 0, 0, 0};
 
-    float mx = 0.f;
+    constexpr bool U8 = std::is_same_v<IN, uint8_t>;
+    // Scale exponent: the largest scaled input in [2^12, 2^13).  u8 input
+    // takes the one for 255 (2^5) whatever the strip holds: every byte
+    // times 2^5 mq is exact in f16, and the error bound, relative to 255,
+    // stays that of the brightest strip.
+    int ex = 5;
+    if constexpr (!U8) {
+        float mx = 0.f;
 #pragma unroll
-    for (int b = 0; b < NVB; ++b)
+        for (int b = 0; b < NVB; ++b)
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) mx = fmaxf(mx, fabsf(raw[b][j]));
-    mx = wave_max(mx);
-    int ex;
-    (void)frexpf(mx, &ex);                      // mx < 2^ex
-    ex = 13 - ex;                               // largest scaled input in [2^12, 2^13)
+            for (int j = 0; j < NLD; ++j) mx = fmaxf(mx, fabsf(raw_value<IN>(raw[b][j])));
+        mx = wave_max(mx);
+        (void)frexpf(mx, &ex);                  // mx < 2^ex
+        ex = 13 - ex;
+    }
     // Strips whose largest magnitude is below 2^-87 or above 2^113 are out
     // of the scale's reach (their scaled values would lose precision or
     // overflow f16): the FP32 form filters them.  u8 input never is.
-    constexpr bool CHECK = !std::is_same_v<IN, uint8_t> && MODE != MODE_ICIA;
+    constexpr bool CHECK = !U8 && MODE != MODE_ICIA;
     bool redo = CHECK && (ex < -100 || ex > 100);   // wave-uniform
     const float s = ldexpf(a.mq, ex), inv_s = ldexpf(1.f, -ex);
 
@@ -489,17 +591,37 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     // entries 8 .. 11 (rows 32 + 4 g + j), high then low halves, so that one
     // product with the high taps twice covers both halves of those rows
     v8f16 xh[NVB], xl[NVB], xt[NVB];
+    if constexpr (U8) {
+        // s = 2^5 mq lies in [2^-7, 2^5] (mq in [2^-12, 1]), so M = 1024 s
+        // is a normal f16
+        const _Float16 M = _Float16(1024.f * s);
+        const uint32_t mb = uint32_t(__builtin_bit_cast(uint16_t, M)) * 0x10001u;
+        const v2f16 nm = {-M, -M};
 #pragma unroll
-    for (int b = 0; b < NVB; ++b) {
-        v4f16 h0, l0, h1, l1;
-        split4(raw[b][0], raw[b][1], raw[b][2], raw[b][3], s, h0, l0);
-        split4(raw[b][4], raw[b][5], raw[b][6], raw[b][7], s, h1, l1);
-        xh[b] = cat8(h0, h1);
-        xl[b] = cat8(l0, l1);
-        if constexpr (G::K48) {
-            v4f16 h2, l2;
-            split4(raw[b][8], raw[b][9], raw[b][10], raw[b][11], s, h2, l2);
-            xt[b] = cat8(h2, XLO ? l2 : z4);            // 16-bit and u8 inputs: lo is 0
+        for (int b = 0; b < NVB; ++b) {
+            const v2f16 p0 = u8pair_f16(raw[b][0], raw[b][1], mb, nm), p1 = u8pair_f16(raw[b][2], raw[b][3], mb, nm);
+            const v2f16 p2 = u8pair_f16(raw[b][4], raw[b][5], mb, nm), p3 = u8pair_f16(raw[b][6], raw[b][7], mb, nm);
+            xh[b] = v8f16{p0[0], p0[1], p1[0], p1[1], p2[0], p2[1], p3[0], p3[1]};
+            if constexpr (G::K48) {
+                const v2f16 p4 = u8pair_f16(raw[b][8], raw[b][9], mb, nm);
+                const v2f16 p5 = u8pair_f16(raw[b][10], raw[b][11], mb, nm);
+                xt[b] = v8f16{p4[0], p4[1], p5[0], p5[1], 0, 0, 0, 0};
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NVB; ++b) {
+            v4f16 h0, l0, h1, l1;
+            const auto r = [&](int j) { return raw_value<IN>(raw[b][j]); };
+            split4(r(0), r(1), r(2), r(3), s, h0, l0);
+            split4(r(4), r(5), r(6), r(7), s, h1, l1);
+            xh[b] = cat8(h0, h1);
+            xl[b] = cat8(l0, l1);
+            if constexpr (G::K48) {
+                v4f16 h2, l2;
+                split4(r(8), r(9), r(10), r(11), s, h2, l2);
+                xt[b] = cat8(h2, XLO ? l2 : z4);        // 16-bit inputs: lo is 0
+            }
         }
     }
     after_vpass();
@@ -659,17 +781,18 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float*
     if constexpr (CHECK) {
         if (redo) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            mfma_strip_fp32<RC, IN, OUT, MODE>(a, y0, x0, T, stage);
+            mfma_strip_fp32<RC, IN, OUT, MODE>(a, y0, x0, T, stage, mask);
         }
     }
 }
 
-// The same without a sink (every mode that writes its outputs).
+// The same without a sink, for the outputs of a.mask (every mode that
+// writes its outputs).
 template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
 __device__ __forceinline__ void
-mfma_compute(const A& a, int y0, int x0, MfmaRaw& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
+mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
 {
-    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T, stage, after_vpass, 0);
+    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T, stage, after_vpass, 0, a.mask);
 }
 #undef TUCU_MFMA
 
@@ -713,7 +836,7 @@ sepfir_mfma_kernel(const SepArgs a)
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
     int y0, x0;
     const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
-    MfmaRaw raw;
+    MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
     mfma_build_tabs<MODE>(a, T);
     __syncthreads();
@@ -744,7 +867,7 @@ sepfir_mfma_loop_kernel(const SepArgs a, int ntiles)
     float* const stage = reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH;
     int y0, x0;
     bool live = t < hi && mfma_strip_pos<WY>(a, t, y0, x0);
-    MfmaRaw raw;
+    MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
     mfma_build_tabs<MODE>(a, T);
     __syncthreads();
@@ -810,7 +933,7 @@ batch_mfma_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __restrict
             }
             int y0, x0;
             if (mfma_strip_pos<WY>(a, int(e.tile), y0, x0)) {
-                MfmaRaw raw;
+                MfmaRaw<IN> raw;
                 if (a.rt <= 8) {
                     mfma_load<8, IN>(a, y0, x0, raw);
                     mfma_compute<8, IN, OUT>(a, y0, x0, raw, T, stage, [] {});
@@ -858,7 +981,7 @@ batch_mfma_wave_kernel(const SepArgs* __restrict__ jobs_, const BatchTile* __res
             const int y0 = __builtin_amdgcn_readfirstlane(a.cy0 + 16 * (int(e.tile) - tx * a.tiles_y));
             const int x0 = __builtin_amdgcn_readfirstlane(a.cx0 + MSTRIP_W * tx);
             if (y0 < a.cy1 && x0 < a.cx1) {
-                MfmaRaw raw;
+                MfmaRaw<IN> raw;
                 float* const st = reinterpret_cast<float*>(stage4);
                 if (a.rt <= 8) {
                     mfma_load<8, IN>(a, y0, x0, raw);
@@ -932,7 +1055,7 @@ icia_mfma_kernel(const SepArgs a, double* __restrict__ partial)
     __shared__ float red[MWAVES][NM];
     int y0, x0;
     const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
-    MfmaRaw raw;
+    MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);
     mfma_build_tabs<MODE_GAUSS>(a, T);
     __syncthreads();
@@ -949,7 +1072,7 @@ icia_mfma_kernel(const SepArgs a, double* __restrict__ partial)
                     if (k == 1) gH[o] = acc[o] * u;
                     else        gV[o] = acc[o] * u;
                 }
-            });
+            }, a.mask);
         // lane (g, m): row y0 + m, columns x0 + 16 o + 4 g + i of the accumulators
         const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
         const float sc = 1.f / float(max(a.nrow, a.ncol));
