@@ -9,6 +9,7 @@
 #include "tucu.h"
 #include "sepfir.h"
 #include "mfma.h"
+#include "icia.h"
 #include "subsample.h"
 
 #include <hip/hip_runtime.h>
@@ -17,7 +18,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 namespace tucu {
 
@@ -96,6 +99,7 @@ bool fir_use_mfma()
 }
 
 int launch(const KernelEntry& k, SepArgs& a, hipStream_t s);
+const void* mfma_tabs_for(const SepArgs& a, int mode, hipStream_t s);
 
 // One-output matrix-core launch: full taps vt / ht (tap o at index o + 16)
 // go to a.tv / a.th with their power-of-two scales (mfma.h, MODE_TAPS).
@@ -119,6 +123,7 @@ int launch_fir_mfma(SepArgs& a, const float* vt, const float* ht, int rV, int rH
     a.mp[2] = 1.f;
     a.mq = std::ldexp(1.f, -eq);
     a.mu[0] = std::ldexp(1.f, -(e[0] + e[1] - eq));
+    a.tabs = mfma_tabs_for(a, MODE_TAPS, s);
     return launch(g_fir_mfma_table[std::max(rV, rH) <= 8 ? 0 : 1][in_dt][out_dt], a, s);
 }
 
@@ -198,6 +203,121 @@ void mfma_scales(SepArgs& a)
         const int h = HV[k][0], v = HV[k][1];
         a.mu[k] = std::ldexp(1.f, -(ep[h] + ep[v] - emax));
     }
+}
+
+// Host image of the operand tables the matrix-core kernels build in LDS
+// (mfma_build_tabs in mfma.h, restated step by step: the same float
+// products and the same round-to-nearest-even f16 conversions, so the
+// bytes are identical).
+void mfma_host_tabs(const SepArgs& a, int mode, MfmaTabs& T)
+{
+    const bool taps = mode == MODE_TAPS;
+    const int nl = taps ? 2 : 3, nw = taps ? 66 : 51;
+    std::memset(&T, 0, sizeof(T));
+    for (int i = 0; i < nw; ++i) T.lobe[i] = taps ? (i < 33 ? a.tv[i] : a.th[i - 33]) : a.cv[i / 17][i % 17];
+    const auto tap = [&](int l, int o) -> float {
+        const int k = o < 0 ? -o : o;
+        if (k > 16) return 0.f;
+        if (taps) return l < 2 ? T.lobe[33 * l + o + 16] : 0.f;
+        const float c = T.lobe[17 * l + k];
+        return l != 1 ? c : o == 0 ? 0.f : o < 0 ? c : -c;
+    };
+    constexpr int NT4 = MT4 + 4;
+    for (int l = 0; l < nl; ++l)
+        for (int t = 0; t < NT4; ++t) T.tap[l][t] = tap(l, t - MBASE) * a.mp[l];
+    for (int e = 0; e < nl * (MT4 + MHP); ++e) {
+        const bool pair = e >= nl * MT4;
+        const int ee = pair ? e - nl * MT4 : e;
+        const int n = pair ? MHP : MT4;
+        const int l = ee / n, t = ee - l * n;
+        v8f16 hi, lo;
+        for (int i = 0; i < 8; ++i) {
+            const float c = i < 4 || pair ? T.tap[l][t + (i < 4 ? i : 12 + i)] : 0.f;
+            const _Float16 h = _Float16(c);
+            hi[i] = h;
+            lo[i] = _Float16(c - float(h));
+        }
+        if (pair) {
+            T.hpair[(0 * 3 + l) * MHP + t] = hi;
+            T.hpair[(1 * 3 + l) * MHP + t] = lo;
+        } else {
+            T.run[(0 * 3 + l) * MT4 + t] = v4f16{hi[0], hi[1], hi[2], hi[3]};
+            T.run[(1 * 3 + l) * MT4 + t] = v4f16{lo[0], lo[1], lo[2], lo[3]};
+        }
+    }
+}
+
+// Prebuilt operand tables in device memory, one per distinct (mode, lobe
+// words, tap scales), at most TAB_SLOTS per device; the slots are never
+// reused or freed (they live until the process ends), so a table a launch
+// points at stays valid for every later replay of a captured graph.  A new
+// table is uploaded on the caller's stream, which is then synchronised, so
+// that launches on any stream may use it; under stream capture no table
+// is made (a launch captured before its lobes were seen builds its tables
+// itself), and neither when the slots are used up or memory is short.
+constexpr int TAB_SLOTS = 64;
+
+struct TabCache {
+    std::mutex mu;
+    struct Dev {
+        char* arena = nullptr;
+        MfmaTabs* host = nullptr;
+        bool failed = false;
+        std::unordered_map<std::string, int> slot;
+    };
+    std::unordered_map<int, Dev> dev;
+};
+
+TabCache& tab_cache()
+{
+    static TabCache* c = new TabCache;          // never destroyed: launches may run at exit
+    return *c;
+}
+
+const void* mfma_tabs_for(const SepArgs& a, int mode, hipStream_t s)
+{
+    const char* env = std::getenv("TUCU_PREBUILT_TABLES");   // "0": every workgroup builds them (read per call)
+    if (env && std::strcmp(env, "0") == 0) return nullptr;
+    const bool taps = mode == MODE_TAPS;
+    std::string key(1, char(mode));
+    const int nw = taps ? 33 : 51;
+    if (taps) {
+        key.append(reinterpret_cast<const char*>(a.tv), nw * sizeof(float));
+        key.append(reinterpret_cast<const char*>(a.th), nw * sizeof(float));
+    } else {
+        key.append(reinterpret_cast<const char*>(a.cv), nw * sizeof(float));
+    }
+    key.append(reinterpret_cast<const char*>(a.mp), sizeof(a.mp));
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) return nullptr;
+    TabCache& c = tab_cache();
+    std::lock_guard<std::mutex> lock(c.mu);
+    TabCache::Dev& D = c.dev[d];
+    const auto it = D.slot.find(key);
+    if (it != D.slot.end()) return D.arena + size_t(it->second) * sizeof(MfmaTabs);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    if (D.failed || int(D.slot.size()) >= TAB_SLOTS) return nullptr;
+    if (!D.arena) {
+        void* p = nullptr;
+        if (hipMalloc(&p, TAB_SLOTS * sizeof(MfmaTabs)) != hipSuccess) {
+            (void)hipGetLastError();
+            D.failed = true;
+            return nullptr;
+        }
+        D.arena = static_cast<char*>(p);
+        D.host = new MfmaTabs[TAB_SLOTS];
+    }
+    const int k = int(D.slot.size());
+    mfma_host_tabs(a, mode, D.host[k]);
+    char* const dst = D.arena + size_t(k) * sizeof(MfmaTabs);
+    if (hipMemcpyAsync(dst, &D.host[k], sizeof(MfmaTabs), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    D.slot.emplace(std::move(key), k);
+    return dst;
 }
 
 // Output k of the kernel: base address of centre (0, 0) and row stride.
@@ -656,6 +776,7 @@ int tucu_gauss2_convolve(const float* lobe0, int n0, const float* lobe1, int n1,
         const int rc = rmax <= 8 ? 0 : 1;
         const KernelEntry* m = &g_gauss_mfma_table[rc][mfma_wy_index()][in->dtype][odt];
         if (!m->fn) m = &g_gauss_mfma_table[rc][MFMA_WY_DEFAULT][in->dtype][odt];
+        a.tabs = mfma_tabs_for(a, MODE_GAUSS, st);
         return copy.finish(launch(*m, a, st), st);
     }
     const KernelEntry* e = &g_gauss_table[gauss_class(rmax)][gauss_th_index()][in->dtype][odt];
@@ -805,6 +926,7 @@ int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, con
         mfma_scales(a);
         a.mask = live;
         a.rt = std::max(r0, r1) <= 8 ? 8 : 16;
+        a.tabs = mfma_tabs_for(a, MODE_GAUSS, static_cast<hipStream_t>(stream));
     }
     double* partial = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(workspace) + 255) / 256 * 256);
     const IciaLaunchFn fn = g_icia_table[std::max(r0, r1) <= 8 ? 0 : 1][in->dtype][model == TUCU_ICIA_AFFINE ? 0 : 1];

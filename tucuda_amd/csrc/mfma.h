@@ -80,7 +80,7 @@ template <int RC> struct MGeo {
 // left side +c[k] and the lower / right side -c[k] (FIRFilter.h:66-176 pairs
 // them the same way).  A lane's operand is one or two runs, so it loads
 // with 8-byte LDS reads and needs no conversion.
-struct MfmaTabs {
+struct alignas(16) MfmaTabs {
     v4f16 run[2 * 3 * MT4];
     v8f16 hpair[2 * 3 * MHP];  // horizontal operands: runs t and t + 16 side by side
     float lobe[2 * 33];        // staging: cv[l][k] at 17 l + k (GAUSS), tv / th at 33 l + o + 16 (TAPS)
@@ -165,6 +165,23 @@ __device__ __forceinline__ void mfma_build_tabs(const A& a, MfmaTabs& T)
             T.run[(0 * 3 + l) * MT4 + t] = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
             T.run[(1 * 3 + l) * MT4 + t] = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
         }
+    }
+}
+
+// The operand tables of a launch in LDS: copied from a.tabs, where the
+// host has built them once for these lobes (the same values, bit for bit:
+// mfma_host_tabs restates mfma_build_tabs), else built here.  Every thread
+// of the workgroup calls it; the caller puts a barrier between it and the
+// first use of T.  (Building them took 2-3 us of a 1080p launch.)
+template <int MODE = MODE_GAUSS, class A>
+__device__ __forceinline__ void mfma_get_tabs(const A& a, MfmaTabs& T)
+{
+    if (a.tabs) {                               // uniform
+        const gptr<const v4f32> src = reinterpret_cast<gptr<const v4f32>>(reinterpret_cast<uintptr_t>(a.tabs));
+        v4f32* const dst = reinterpret_cast<v4f32*>(&T);
+        for (int i = threadIdx.x; i < int(sizeof(MfmaTabs) / 16); i += blockDim.x) dst[i] = src[i];
+    } else {
+        mfma_build_tabs<MODE>(a, T);
     }
 }
 
@@ -838,7 +855,7 @@ sepfir_mfma_kernel(const SepArgs a)
     const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
     MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
-    mfma_build_tabs<MODE>(a, T);
+    mfma_get_tabs<MODE>(a, T);
     __syncthreads();
     if (!live) return;
     mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T,
@@ -869,7 +886,7 @@ sepfir_mfma_loop_kernel(const SepArgs a, int ntiles)
     bool live = t < hi && mfma_strip_pos<WY>(a, t, y0, x0);
     MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
-    mfma_build_tabs<MODE>(a, T);
+    mfma_get_tabs<MODE>(a, T);
     __syncthreads();
     while (t < hi) {                                // uniform over the workgroup
         if (live) mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T, stage, [] {});
@@ -1022,136 +1039,7 @@ constexpr int NMWY = 3;
 constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
 constexpr int MFMA_WY_DEFAULT = 1;          // index: 2 x 2 strips, 32 x 128 pixels
 
-// ------------------------------------------------------------------------
-// ICIA's error moment (TU/cu/ICIA.h:346-369 with ICIAErrorMoment, :61-112):
-// diffH and diffV of the image from one read (the fused pass, mask 0x06),
-// and, without writing them, the sum over every pixel of the upper triangle
-// of a a^T, a = image_derivative0(s u, s v, eH, eV) of the affine (D = 6,
-// TU/cu/vec.h:1418-1422) or projective (D = 8, :1285-1290) map, s = 1 /
-// max(nrow, ncol), u, v the pixel's column and row.  A gradient is 0 outside
-// its valid region, as the reference's freshly allocated gradient images
-// are there.  Each workgroup sums its tile (FP32 per lane, FP64 across lanes
-// and waves) into partial[blockIdx.x]; icia_reduce_kernel adds the partials
-// in a fixed order.
-
-// Sum over the wave, the result in every lane.
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-template <int D> constexpr int icia_nm() { return D * (D + 1) / 2; }
-
-template <int RC, class IN, int D>
-__global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(MFMA_WAVES_PER_SIMD<IN>)))
-icia_mfma_kernel(const SepArgs a, double* __restrict__ partial)
-{
-    constexpr int WY = MFMA_WYS[MFMA_WY_DEFAULT];
-    constexpr int NM = icia_nm<D>();
-    __shared__ MfmaTabs T;
-    __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
-    __shared__ float red[MWAVES][NM];
-    int y0, x0;
-    const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
-    MfmaRaw<IN> raw;
-    if (live) mfma_load<RC, IN>(a, y0, x0, raw);
-    mfma_build_tabs<MODE_GAUSS>(a, T);
-    __syncthreads();
-    float mom[NM];
-#pragma unroll
-    for (int k = 0; k < NM; ++k) mom[k] = 0.f;
-    if (live) {
-        v4f32 gH[MNB], gV[MNB];
-        mfma_compute<RC, IN, float, MODE_ICIA>(
-            a, y0, x0, raw, T, reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH, [] {},
-            [&](int k, const v4f32* acc, float u) {
-#pragma unroll
-                for (int o = 0; o < MNB; ++o) {
-                    if (k == 1) gH[o] = acc[o] * u;
-                    else        gV[o] = acc[o] * u;
-                }
-            }, a.mask);
-        // lane (g, m): row y0 + m, columns x0 + 16 o + 4 g + i of the accumulators
-        const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
-        const float sc = 1.f / float(max(a.nrow, a.ncol));
-        const int y = y0 + m, r0 = a.r[0], r1 = a.r[1];
-        const bool rowH = y >= r0 && y < a.nrow - r0;          // diffH: V lobe 0, H lobe 1
-        const bool rowV = y >= r1 && y < a.nrow - r1;          // diffV: V lobe 1, H lobe 0
-        const float vf = sc * float(y);
-#pragma unroll
-        for (int o = 0; o < MNB; ++o)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = x0 + 16 * o + 4 * g + i;
-                const float eH = rowH && x >= r1 && x < a.ncol - r1 ? gH[o][i] : 0.f;
-                const float eV = rowV && x >= r0 && x < a.ncol - r0 ? gV[o][i] : 0.f;
-                const float uf = sc * float(x);
-                float av[D];
-                av[0] = eH * uf; av[1] = eH * vf; av[2] = eH;
-                av[3] = eV * uf; av[4] = eV * vf; av[5] = eV;
-                if constexpr (D == 8) {
-                    const float t = -(eH * uf + eV * vf);
-                    av[6] = t * uf;
-                    av[7] = t * vf;
-                }
-                int k = 0;
-#pragma unroll
-                for (int p = 0; p < D; ++p)
-#pragma unroll
-                    for (int q = p; q < D; ++q, ++k) mom[k] = __builtin_fmaf(av[p], av[q], mom[k]);
-            }
-    }
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-        const float t = wave_sum(mom[k]);
-        if ((threadIdx.x & 63) == 0) red[w][k] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < NM) {
-        double t = 0.0;
-#pragma unroll
-        for (int v = 0; v < MWAVES; ++v) t += double(red[v][threadIdx.x]);
-        partial[int64_t(blockIdx.x) * NM + threadIdx.x] = t;
-    }
-}
-
-// moment[k] = the sum of partial[b][k] over the nblocks workgroups, in FP64,
-// in a fixed order (a strided sum per thread, then a tree); 0 when nblocks is 0.
-template <int NM>
-__global__ void __launch_bounds__(256) icia_reduce_kernel(const double* __restrict__ partial, int nblocks,
-                                                          float* __restrict__ moment)
-{
-    __shared__ double sh[256];
-    for (int k = 0; k < NM; ++k) {
-        double t = 0.0;
-        for (int b = int(threadIdx.x); b < nblocks; b += 256) t += partial[int64_t(b) * NM + k];
-        sh[threadIdx.x] = t;
-        __syncthreads();
-        for (int d = 128; d > 0; d >>= 1) {
-            if (int(threadIdx.x) < d) sh[threadIdx.x] += sh[threadIdx.x + d];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) moment[k] = float(sh[0]);
-        __syncthreads();
-    }
-}
-
-template <int RC, class IN, int D>
-hipError_t launch_icia(const SepArgs& a, int nblocks, double* partial, float* moment, hipStream_t s)
-{
-    if (nblocks > 0)
-        hipLaunchKernelGGL((icia_mfma_kernel<RC, IN, D>), dim3(nblocks), dim3(64 * MWAVES), 0, s, a, partial);
-    hipLaunchKernelGGL(icia_reduce_kernel<icia_nm<D>()>, dim3(1), dim3(256), 0, s, partial, nblocks, moment);
-    return hipGetLastError();
-}
-
-using IciaLaunchFn = hipError_t (*)(const SepArgs&, int, double*, float*, hipStream_t);
-
 extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 extern KernelEntry g_fir_mfma_table[2][4][4];       // MODE_TAPS, default shape: [radius class][in][out]
-extern IciaLaunchFn g_icia_table[2][4][2];          // [radius class][in][affine, projective]
 
 }   // namespace tucu

@@ -85,6 +85,8 @@ struct SepArgs {
     uintptr_t obase[6];            // output k: address of the element for centre (0, 0), i.e.
                                    // data + (dy_k * stride + dx_k) * elem (may lie before data)
     int32_t  ostride[6];           // output row stride, elements
+    const void* tabs;              // matrix-core kernels: the operand tables (MfmaTabs) prebuilt in
+                                   // device memory, or null: each workgroup builds them
 };
 
 // Job descriptors read by the batched kernel live in device memory the
