@@ -118,7 +118,7 @@ __device__ __forceinline__ void icia_lane_moments(const SepArgs& a, int y0, int 
 // Workgroup b filters tiles t, t + grid, t + 2 grid, ... with t =
 // xcd_remap(b) (strip-major tile numbers: the workgroups of one XCD take
 // contiguous ranges, whose tiles share halo rows in its L2) and writes its
-// FP64 partial moments to partial[b].
+// FP64 partial moments to partial[k][b] (k = 0 .. NM - 1).
 template <int RC, class IN, int D>
 __global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))   // 3 spill
 icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
@@ -171,31 +171,34 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
         for (int v = 0; v < MWAVES; ++v)
 #pragma unroll
             for (int r = 0; r < 4; ++r) s += double(red[v][r][threadIdx.x]);
-        partial[int64_t(blockIdx.x) * NM + threadIdx.x] = s;
+        partial[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = s;   // moment-major
     }
 }
 
-// moment[k] = the sum of partial[b][k] over the nblocks workgroups, in FP64
+// moment[k] = the sum of partial[k][b] over the nblocks workgroups, in FP64
 // in a fixed order (thread t adds b = t, t + 128, ..., then thread k adds
-// the 128 thread sums in order); 0 when nblocks is 0.
+// the 128 thread sums in order); 0 when nblocks is 0.  The partials are
+// stored moment-major, so that each of the loads here reads 128
+// consecutive doubles.
 template <int NM>
 __global__ void __launch_bounds__(128) icia_reduce_kernel(const double* __restrict__ partial, int nblocks,
                                                           float* __restrict__ moment)
 {
-    __shared__ double sh[128][NM];
+    __shared__ double sh[NM][129];
+    const int t = int(threadIdx.x);
     double acc[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) acc[k] = 0.0;
-    for (int b = int(threadIdx.x); b < nblocks; b += 128)
+    for (int b = t; b < nblocks; b += 128)
 #pragma unroll
-        for (int k = 0; k < NM; ++k) acc[k] += partial[int64_t(b) * NM + k];
+        for (int k = 0; k < NM; ++k) acc[k] += partial[int64_t(k) * nblocks + b];
 #pragma unroll
-    for (int k = 0; k < NM; ++k) sh[threadIdx.x][k] = acc[k];
+    for (int k = 0; k < NM; ++k) sh[k][t] = acc[k];
     __syncthreads();
-    if (threadIdx.x < NM) {
+    if (t < NM) {
         double s = 0.0;
-        for (int i = 0; i < 128; ++i) s += sh[i][threadIdx.x];
-        moment[threadIdx.x] = float(s);
+        for (int i = 0; i < 128; ++i) s += sh[t][i];
+        moment[t] = float(s);
     }
 }
 
