@@ -88,3 +88,52 @@ def test_icia_prebuilt_tables_bitwise(dev, model, sigma):
     x = to_dev(synth.edge_image(12, 257, 391), dev)
     a, b = _both(lambda: tucuda_amd.icia_moment(sigma, x, model))
     assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_more_lobe_sets_than_table_slots(dev):
+    """Past the 64 table slots of a device, launches build their tables in
+    every workgroup: 70 distinct sigmas, each result equal to the in-kernel
+    build's bits."""
+    import torch
+    import tucuda_amd
+    x = to_dev(synth.edge_image(13, 96, 200), dev)
+    for i in range(70):
+        sigma = 0.8 + 0.047 * i
+        g = tucuda_amd.FIRGaussianConvolver2(sigma)
+
+        def run():
+            o = torch.zeros_like(x)
+            g.smooth(x, o)
+            return o
+        a, b = _both(run)
+        assert np.array_equal(_bits(a), _bits(b)), sigma
+
+
+def test_graph_captured_before_the_lobes_were_seen(dev):
+    """A launch captured into a HIP graph before its lobes were ever used
+    gets no prebuilt table (nothing is uploaded under capture) and builds
+    its tables itself: every replay equals the eager launch's bits, with
+    eager work in between."""
+    import torch
+    import tucuda_amd
+    x = to_dev(synth.edge_image(14, 130, 260), dev)
+    g = tucuda_amd.FIRGaussianConvolver2(2.71828)      # a sigma no other test uses
+    out = torch.zeros_like(x)
+    st = torch.cuda.Stream(dev)
+    st.wait_stream(torch.cuda.current_stream(dev))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(st):
+        with torch.cuda.graph(graph, stream=st):
+            g.convolve6(x, {"smooth": out})
+    torch.cuda.current_stream(dev).wait_stream(st)
+    ref = torch.zeros_like(x)
+    g.convolve6(x, {"smooth": ref})                  # eager: uploads the table now
+    torch.cuda.synchronize()
+    for _ in range(3):
+        out.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(out), _bits(ref))
+        tmp = torch.zeros_like(x)
+        g.convolve6(x, {"diffH": tmp})               # eager work between replays
+        torch.cuda.synchronize()
