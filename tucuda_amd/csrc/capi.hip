@@ -364,6 +364,8 @@ int finish_args(SepArgs& a, const tucu_image& in, unsigned live, const tucu_imag
         a.fx0 = std::max(a.fx0, rh[k]); a.fx1 = std::min(a.fx1, in.ncol - rh[k]);
         const uintptr_t oes = dtype_size(outs[k].dtype);
         if (a.ostride[k] % 4 != 0 || (a.obase[k] + uintptr_t(a.cx0) * oes) % (4 * oes) != 0) aligned = false;
+        // the fast stores address 16 rows from a strip's first row in 32 bits
+        if ((int64_t(16) * a.ostride[k] + in.ncol + 256) * int64_t(oes) >= (int64_t(1) << 31)) aligned = false;
     }
     if (!aligned) a.fy0 = a.fy1 = a.fx0 = a.fx1 = 0;
     return TUCU_OK;

@@ -566,7 +566,7 @@ __device__ __forceinline__ void mfma_load(const A& a, int y0, int x0, MfmaRaw<IN
 //   horizontal, K48: two K32 slices over the fixed pairs P_(o/2) and
 //     P_(o/2 + 1) of field blocks (2 p, 2 p + 1); A = taps as above, shifted
 //     by d = 32 p - 16 o (zero for the block outside the window).
-template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F, class S>
+template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, int CPOL = CPOL_NT, class A, class F, class S>
 __device__ __forceinline__ void
 mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, float* stage, F&& after_vpass,
              S&& sink, unsigned mask)
@@ -758,13 +758,30 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
             for (int o = 0; o < MNB; ++o)
                 *reinterpret_cast<v4f32*>(stage + m * MOPITCH + 16 * o + 4 * g) = acc[o] * u;
             __builtin_amdgcn_wave_barrier();
+            if constexpr (CPOL == CPOL_NT) {
 #pragma unroll
-            for (int o = 0; o < MNB; ++o) {
-                const int row = 4 * o + g;
-                const v4f32 q = *reinterpret_cast<const v4f32*>(stage + row * MOPITCH + 4 * m);
-                const gptr<OUT> p = reinterpret_cast<gptr<OUT>>(
-                    a.obase[k] + uintptr_t(int64_t(y0 + row) * a.ostride[k] + x0 + 4 * m) * sizeof(OUT));
-                Px<OUT>::st4nt(p, q[0], q[1], q[2], q[3]);    // written once: streaming stores
+                for (int o = 0; o < MNB; ++o) {
+                    const int row = 4 * o + g;
+                    const v4f32 q = *reinterpret_cast<const v4f32*>(stage + row * MOPITCH + 4 * m);
+                    const gptr<OUT> p = reinterpret_cast<gptr<OUT>>(
+                        a.obase[k] + uintptr_t(int64_t(y0 + row) * a.ostride[k] + x0 + 4 * m) * sizeof(OUT));
+                    Px<OUT>::st4nt(p, q[0], q[1], q[2], q[3]);    // written once: streaming stores
+                }
+            } else {
+                // buffer stores carry the cache policy: rows from the strip's
+                // first row (a uniform base), lane offsets in 32 bits (the host
+                // keeps 16 rows of every output within 2^31 bytes, or no strip
+                // takes this path)
+                const __amdgpu_buffer_rsrc_t orsrc = __builtin_amdgcn_make_buffer_rsrc(
+                    reinterpret_cast<void*>(a.obase[k] + uintptr_t(int64_t(y0) * a.ostride[k]) * sizeof(OUT)),
+                    short(0), int(-1), BUF_DWORD3);
+#pragma unroll
+                for (int o = 0; o < MNB; ++o) {
+                    const int row = 4 * o + g;
+                    const v4f32 q = *reinterpret_cast<const v4f32*>(stage + row * MOPITCH + 4 * m);
+                    const uint32_t off = uint32_t((row * a.ostride[k] + x0 + 4 * m) * int(sizeof(OUT)));
+                    Px<OUT>::template st4buf<CPOL>(orsrc, off, q[0], q[1], q[2], q[3]);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         } else {
@@ -805,11 +822,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
 
 // The same without a sink, for the outputs of a.mask (every mode that
 // writes its outputs).
-template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, class A, class F>
+template <int RC, class IN, class OUT, int MODE = MODE_GAUSS, int CPOL = CPOL_NT, class A, class F>
 __device__ __forceinline__ void
 mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, float* stage, F&& after_vpass)
 {
-    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T, stage, after_vpass, 0, a.mask);
+    mfma_compute<RC, IN, OUT, MODE, CPOL>(a, y0, x0, raw, T, stage, after_vpass, 0, a.mask);
 }
 #undef TUCU_MFMA
 
@@ -858,8 +875,10 @@ sepfir_mfma_kernel(const SepArgs a)
     mfma_get_tabs<MODE>(a, T);
     __syncthreads();
     if (!live) return;
-    mfma_compute<RC, IN, OUT, MODE>(a, y0, x0, raw, T,
-                                    reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH, [] {});
+    // one round of workgroups: outputs written through L2 (CPOL_NT_WT)
+    mfma_compute<RC, IN, OUT, MODE, CPOL_NT_WT>(a, y0, x0, raw, T,
+                                                reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH,
+                                                [] {});
 }
 
 // One large image: a grid of the resident workgroups; each builds the tap

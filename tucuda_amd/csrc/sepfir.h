@@ -95,6 +95,17 @@ struct SepArgs {
 // the tile loops), as it does for kernel arguments.
 typedef const __attribute__((address_space(4))) SepArgs ConstArgs;
 
+// Cache policies of the matrix-core kernels' output stores (buffer
+// instruction bits: nt = 2, sc1 = 16).  CPOL_NT: streaming.  CPOL_NT_WT:
+// streaming and at agent scope, so written through the XCD's L2: the
+// kernel-end release then finds no dirty output lines to write back, which
+// takes 4 % off a one-round launch (1080p six outputs) but costs 3-7 % in
+// the tile loops, where the L2 absorbs the stores of many rounds.
+constexpr int CPOL_NT = 2;
+constexpr int CPOL_NT_WT = 2 | 16;
+
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
 // ----------------------------------------------------------------- pixels
 template <class T> struct Px;
 
@@ -111,6 +122,11 @@ template <> struct Px<float> {
     static __device__ __forceinline__ void st4nt(gptr<float> p, float a, float b, float c, float d)
     {
         __builtin_nontemporal_store(v4f32{a, b, c, d}, reinterpret_cast<gptr<v4f32>>(p));
+    }
+    template <int CP>
+    static __device__ __forceinline__ void st4buf(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c, float d)
+    {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u32, v4f32{a, b, c, d}), r, int(off), 0, CP);
     }
 };
 
@@ -133,6 +149,11 @@ template <> struct Px<uint8_t> {
     {
         __builtin_nontemporal_store(cvt(a) | cvt(b) << 8 | cvt(c) << 16 | cvt(d) << 24, reinterpret_cast<gptr<uint32_t>>(p));
     }
+    template <int CP>
+    static __device__ __forceinline__ void st4buf(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c, float d)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(cvt(a) | cvt(b) << 8 | cvt(c) << 16 | cvt(d) << 24, r, int(off), 0, CP);
+    }
 };
 
 template <> struct Px<f16_t> {
@@ -148,6 +169,11 @@ template <> struct Px<f16_t> {
     static __device__ __forceinline__ void st4nt(gptr<f16_t> p, float a, float b, float c, float d)
     {
         __builtin_nontemporal_store(v2u32{pack_f16(a, b), pack_f16(c, d)}, reinterpret_cast<gptr<v2u32>>(p));
+    }
+    template <int CP>
+    static __device__ __forceinline__ void st4buf(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c, float d)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(v2u32{pack_f16(a, b), pack_f16(c, d)}, r, int(off), 0, CP);
     }
 };
 
@@ -167,6 +193,11 @@ template <> struct Px<bf16_t> {
     static __device__ __forceinline__ void st4nt(gptr<bf16_t> p, float a, float b, float c, float d)
     {
         __builtin_nontemporal_store(v2u32{pack_bf16(a, b), pack_bf16(c, d)}, reinterpret_cast<gptr<v2u32>>(p));
+    }
+    template <int CP>
+    static __device__ __forceinline__ void st4buf(__amdgpu_buffer_rsrc_t r, uint32_t off, float a, float b, float c, float d)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(v2u32{pack_bf16(a, b), pack_bf16(c, d)}, r, int(off), 0, CP);
     }
 };
 
