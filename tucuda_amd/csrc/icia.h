@@ -128,6 +128,7 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
     __shared__ float red[MWAVES][4][NM];
+    mfma_args_early(a);
     float* const stage = reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH;
     const int grid = int(gridDim.x);
     int t = xcd_remap(int(blockIdx.x), grid);

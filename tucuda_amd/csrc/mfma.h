@@ -860,6 +860,24 @@ __device__ __forceinline__ int mfma_share_lo(int ntiles, int grid, int x)
     return int(int64_t(ntiles) * (q * x + min(x, r)) / grid);
 }
 
+// Reads, in one round of scalar loads, every kernel argument a strip's
+// loads depend on.  Left to itself the compiler reads each where it is first
+// used, behind branches that need the one before: six dependent round trips
+// to the argument block before a wave could issue its first input load
+// (1080p six outputs: 2 % of the launch).
+template <class A>
+__device__ __forceinline__ void mfma_args_early(const A& a)
+{
+    const int grid = int(gridDim.x);
+    const int ty = a.tiles_y, cy0 = a.cy0, cy1 = a.cy1, cx0 = a.cx0, cx1 = a.cx1;
+    const int nr = a.nrow, nc = a.ncol, sb = a.in_stride_b;
+    const uint32_t nb = a.in_bytes;
+    const void* const in = a.in;
+    const void* const tabs = a.tabs;
+    asm volatile("" ::"s"(grid), "s"(ty), "s"(cy0), "s"(cy1), "s"(cx0), "s"(cx1), "s"(nr), "s"(nc), "s"(sb), "s"(nb),
+                 "s"(in), "s"(tabs));
+}
+
 // One image, one workgroup per tile (images with no more tiles than
 // resident workgroups, such as 1080p).
 template <int RC, int WY, class IN, class OUT, int MODE = MODE_GAUSS>
@@ -868,6 +886,7 @@ sepfir_mfma_kernel(const SepArgs a)
 {
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    mfma_args_early(a);
     int y0, x0;
     const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
     MfmaRaw<IN> raw;
@@ -896,6 +915,7 @@ sepfir_mfma_loop_kernel(const SepArgs a, int ntiles)
 {
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+    mfma_args_early(a);
     const int grid = int(gridDim.x), b = int(blockIdx.x);
     const int x = b % QN, step = (grid - x + QN - 1) / QN;           // workgroups on this XCD
     const int hi = mfma_share_lo(ntiles, grid, x + 1);
