@@ -606,7 +606,10 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
 
     // xh / xl: entries 0 .. 7 (rows 8 g + j), high and low halves; xt (K = 48):
     // entries 8 .. 11 (rows 32 + 4 g + j), high then low halves, so that one
-    // product with the high taps twice covers both halves of those rows
+    // product with the high taps twice covers both halves of those rows.
+    // Inputs exact in f16 have no low half: xt then holds the high halves
+    // twice, and one product with the high and the low taps side by side
+    // covers both tap halves (one product per block instead of two)
     v8f16 xh[NVB], xl[NVB], xt[NVB];
     if constexpr (U8) {
         // s = 2^5 mq lies in [2^-7, 2^5] (mq in [2^-12, 1]), so M = 1024 s
@@ -622,7 +625,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
             if constexpr (G::K48) {
                 const v2f16 p4 = u8pair_f16(raw[b][8], raw[b][9], mb, nm);
                 const v2f16 p5 = u8pair_f16(raw[b][10], raw[b][11], mb, nm);
-                xt[b] = v8f16{p4[0], p4[1], p5[0], p5[1], 0, 0, 0, 0};
+                xt[b] = v8f16{p4[0], p4[1], p5[0], p5[1], p4[0], p4[1], p5[0], p5[1]};
             }
         }
     } else {
@@ -637,7 +640,7 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
             if constexpr (G::K48) {
                 v4f16 h2, l2;
                 split4(r(8), r(9), r(10), r(11), s, h2, l2);
-                xt[b] = cat8(h2, XLO ? l2 : z4);        // 16-bit inputs: lo is 0
+                xt[b] = cat8(h2, XLO ? l2 : h2);        // 16-bit inputs: lo is 0
             }
         }
     }
@@ -663,8 +666,9 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
         if constexpr (G::K48) {                 // rows 32 + 4 g + j: high taps for both halves
             const int t2 = MBASE + 32 + 4 * g - RC - m;
             const v4f16 th = T.run[(0 * 3 + l) * MT4 + t2];
-            cht = cat8(th, th);
-            clt = cat8(T.run[(1 * 3 + l) * MT4 + t2], z4);
+            const v4f16 tl = T.run[(1 * 3 + l) * MT4 + t2];
+            cht = XLO ? cat8(th, th) : cat8(th, tl);
+            clt = cat8(tl, z4);
         }
         v4f32 acc[NVB];
 #pragma unroll
@@ -677,10 +681,13 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
 #pragma unroll
         for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xh[b], cl1, acc[b]);
         if constexpr (G::K48) {
+            if constexpr (XLO) {
 #pragma unroll
-            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], clt, acc[b]);   // hi x lo taps
+                for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], clt, acc[b]);   // hi x lo taps
+            }
+            // XLO: (hi + lo) x hi taps; otherwise hi x (hi + lo taps)
 #pragma unroll
-            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], cht, acc[b]);   // (hi + lo) x hi taps
+            for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xt[b], cht, acc[b]);
         }
 #pragma unroll
         for (int b = 0; b < NVB; ++b) acc[b] = TUCU_MFMA(xh[b], ch1, acc[b]);
