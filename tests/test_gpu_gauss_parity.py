@@ -111,21 +111,29 @@ def test_workload_1080p_six(dev, kernel_family):
 
 
 @pytest.mark.parametrize("sigma", [1.2, 3.0])
-@pytest.mark.parametrize("ik", ["f32", "u8"])
+@pytest.mark.parametrize("ik", ["f32", "u8", "f16", "bf16"])
 def test_fused_six_large_image(dev, sigma, ik):
     """An image with more tiles than resident workgroups (1512 x 2616, about
     1000 tiles), which the matrix-core path filters with its tile loop (each
-    workgroup builds the tables once and takes a share of the tiles); both
-    radius classes and both input widths, against the oracle."""
+    workgroup copies the tables once and takes a share of the tiles); both
+    radius classes and every input type, against the oracle on the same
+    values as float32 (16-bit values are exact in float32)."""
     import torch
     import tucuda_amd
     img = synth.edge_image(14, 1512, 2616)
+    dev_img = img
     if ik == "u8":
-        img = np.clip(img, 0, 255).astype(np.uint8)
+        img = dev_img = np.clip(img, 0, 255).astype(np.uint8)
+    elif ik == "f16":
+        dev_img = img.astype(np.float16)
+        img = dev_img.astype(np.float32)
+    elif ik == "bf16":
+        dev_img = (img.view(np.uint32) >> 16).astype(np.uint16)      # truncation: exact bf16 values
+        img = (dev_img.astype(np.uint32) << 16).view(np.float32)
     refs = _oracle_all(sigma, img)
     g = tucuda_amd.FIRGaussianConvolver2(sigma)
     outs = {m: torch.full(img.shape, sentinel(np.float32), device=dev) for m in oracle.METHOD_ORDER}
-    g.convolve6(to_dev(img, dev), outs)
+    g.convolve6(to_dev(dev_img, dev), outs)
     for m in oracle.METHOD_ORDER:
         assert_f32_parity(to_host(outs[m]), refs[m], f"large sigma={sigma} {ik} {m}")
 
