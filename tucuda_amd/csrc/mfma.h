@@ -72,6 +72,25 @@ template <int RC> struct MGeo {
     static constexpr int  NLD = K48 ? 12 : 8;             // input values per lane per block
 };
 
+#ifdef TUCU_TIMELINE
+// Phase timeline of the one-round six-output launch (radius class 16, float
+// in and out), in a build made for it (make timeline; read by
+// tucuda_amd/timeline.py): lane 0 of each wave writes the 100 MHz real-time
+// counter into slot i of its row of tucu_timeline at each phase -- 0 start,
+// 10 strip loads issued, 11 operand tables copied, 1 barrier passed, 2
+// input scaled and split, 3 + k output k stored, 9 end.  The default build
+// has none of it.
+constexpr int TL_SLOTS = 12;
+constexpr int TL_WAVES = 2048;
+static __device__ uint64_t tucu_timeline[TL_WAVES * TL_SLOTS];
+__device__ __forceinline__ void tl_stamp(int i)
+{
+    const uint64_t t = __builtin_amdgcn_s_memrealtime();
+    const unsigned w = blockIdx.x * MWAVES + (threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0 && w < unsigned(TL_WAVES)) tucu_timeline[w * TL_SLOTS + i] = t;
+}
+#endif
+
 // Per-lane operand runs, in LDS: run[(part * 3 + l) * MT4 + t] holds taps
 // t - MBASE .. t - MBASE + 3 of lobe l, times mp[l], as four f16 (part 0 the high
 // half, part 1 the low half).  Tap o is the weight of the input at offset o
@@ -798,6 +817,11 @@ mfma_compute(const A& a, int y0, int x0, MfmaRaw<IN>& raw, const MfmaTabs& T, fl
                 store4<OUT>(a, k, false, rv, a.nrow - rv, rh, a.ncol - rh, y, x0 + 16 * o + 4 * g,
                             acc[o][0] * u, acc[o][1] * u, acc[o][2] * u, acc[o][3] * u);
         }
+#ifdef TUCU_TIMELINE
+        if constexpr (CPOL == CPOL_NT_WT && MODE == MODE_GAUSS && RC == 16 && std::is_same_v<IN, float> &&
+                      std::is_same_v<OUT, float>)
+            tl_stamp(3 + k);
+#endif
     };
 
     // One field at a time, each followed by the outputs it feeds, so that
@@ -893,18 +917,35 @@ sepfir_mfma_kernel(const SepArgs a)
 {
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
+#ifdef TUCU_TIMELINE
+    constexpr bool TL = MODE == MODE_GAUSS && RC == 16 && std::is_same_v<IN, float> && std::is_same_v<OUT, float>;
+    const auto stamp = [](int i) {
+        if constexpr (TL) {
+            __builtin_amdgcn_sched_barrier(0);
+            tl_stamp(i);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+#else
+    const auto stamp = [](int) {};
+#endif
+    stamp(0);
     mfma_args_early(a);
     int y0, x0;
     const bool live = mfma_strip_pos<WY>(a, xcd_remap(blockIdx.x, gridDim.x), y0, x0);
     MfmaRaw<IN> raw;
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
+    stamp(10);
     mfma_get_tabs<MODE>(a, T);
+    stamp(11);
     __syncthreads();
+    stamp(1);
     if (!live) return;
     // one round of workgroups: outputs written through L2 (CPOL_NT_WT)
     mfma_compute<RC, IN, OUT, MODE, CPOL_NT_WT>(a, y0, x0, raw, T,
                                                 reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH,
-                                                [] {});
+                                                [&] { stamp(2); });
+    stamp(9);
 }
 
 // One large image: a grid of the resident workgroups; each builds the tap
