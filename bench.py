@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Gpix/s & % HBM roofline (1080p Gauss+5 derivs); p50/p99 ms served stream"
 HBM_PEAK_GBS = 8000.0
 NROW, NCOL, SIGMA = 1080, 1920, 3.0
+LAUNCHES_PER_GRAPH = 50                 # at least this many launches per graph replay
 
 
 def log(*a):
@@ -75,7 +76,7 @@ def cpu_baseline(img, sigma: float, budget_s: float = 12.0, budget_1core_s: floa
             "sample_1core": f"{n1} whole images on 1 thread ({el1:.1f} s)"}
 
 
-def hot_kernel_us(g, ins, outs, launches: int = 10, reps: int = 20) -> float:
+def hot_kernel_us(g, ins, outs, launches: int = LAUNCHES_PER_GRAPH, reps: int = 8) -> float:
     """Microseconds per launch with one buffer set used by every launch (the
     input and outputs stay in the 256 MB Infinity Cache between launches):
     the cache-hot figure beside the rotating one."""
@@ -285,8 +286,10 @@ def served_leg(dev, rates: str, seconds: float) -> dict:
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=3000,
+                    help="untimed steps before the timed ones, run as the timed ones are (graph replays "
+                         "back to back), so that the timed region starts at the GPU's steady clocks")
     ap.add_argument("--sets", type=int, default=10, help="rotating buffer sets (>= 5 defeats the 256 MB MALL)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--persistent", type=int, default=0, metavar="ROWS",
@@ -366,17 +369,24 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    for k in range(args.warmup):
-        ins, outs = sets[k % args.sets]
+    # one call per set from Python first: the operand tables are then
+    # prebuilt when the graph is captured
+    for k in range(min(args.warmup, args.sets)):
+        ins, outs = sets[k]
         g.convolve6(ins, outs)
     torch.cuda.synchronize()
 
-    # Timed region.  Default: one HIP graph holding one launch per buffer
-    # set, replayed until `steps` launches have run (graph replay removes the
-    # per-call host overhead of Python + ctypes).  Events on the launch stream
+    # Timed region.  Default: one HIP graph holding the buffer sets' launches
+    # in turn, at least 50 of them (the sets repeated), replayed until `steps`
+    # launches have run (graph replay removes the per-call host overhead of
+    # Python + ctypes; 50 launches per replay keep the cost of starting a
+    # replay, ~8 us, below 0.2 us per launch).  Events on the launch stream
     # bracket each replay; the kernel-average duration is replay time /
     # launches per replay.
     if args.eager:
+        for k in range(args.sets, args.warmup):
+            ins, outs = sets[k % args.sets]
+            g.convolve6(ins, outs)
         starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         t_start = torch.cuda.Event(enable_timing=True)
@@ -394,17 +404,18 @@ def main():
         kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
         steps = args.steps
     else:
-        per = args.sets
+        per = args.sets * max(1, (LAUNCHES_PER_GRAPH + args.sets - 1) // args.sets)
         graph = torch.cuda.CUDAGraph()
         cap = torch.cuda.Stream(dev)
         cap.wait_stream(stream)
         with torch.cuda.stream(cap):
             with torch.cuda.graph(graph, stream=cap):
                 for k in range(per):
-                    ins, outs = sets[k]
+                    ins, outs = sets[k % args.sets]
                     g.convolve6(ins, outs)
         stream.wait_stream(cap)
-        for _ in range(2):
+        # the untimed steps: replays back to back, as in the timed region
+        for _ in range(max(2, (args.warmup - min(args.warmup, args.sets) + per - 1) // per)):
             graph.replay()
         torch.cuda.synchronize()
         reps = max(1, args.steps // per)

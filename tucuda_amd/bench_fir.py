@@ -1,6 +1,6 @@
 """FIRFilter2 / general-taps and 16-bit storage measurement.
 
-    python -m tucuda_amd.bench_fir [--cases a,b] [--families mfma,valu] [--reps 20]
+    python -m tucuda_amd.bench_fir [--cases a,b] [--families mfma,valu] [--reps 20] [--warmup 40]
 
 Workloads: the 4096x4096 u8 image filtered by 17 x 9 general taps (N(0,1),
 unit L1 norm, seeded) into float32 and into float16, and by FIRFilter2 lobes
@@ -9,7 +9,8 @@ lobes; and the six-output Gaussian (sigma 3) of a 1920x1080 image stored in
 float32, bfloat16 and float16, into outputs of the same type.  Each is timed on
 both kernel families (matrix cores, the default, and the direct FP32 vector
 kernel, TUCU_FIR_PATH / TUCU_GAUSS_PATH=valu), launches replayed from a
-captured HIP graph over 4 rotating buffer sets.  Prints one JSON line with
+captured HIP graph of 48 launches over 4 rotating buffer sets, after
+`--warmup` untimed replays.  Prints one JSON line with
 microseconds per launch and the algorithmic bytes rate (input read once,
 the valid output region of each output written once).
 """
@@ -28,6 +29,8 @@ def main(argv=None):
     ap.add_argument("--cases", default="", help="comma-separated workload names (default: all)")
     ap.add_argument("--families", default="mfma,valu")
     ap.add_argument("--reps", type=int, default=20, help="graph replays timed per workload")
+    ap.add_argument("--warmup", type=int, default=40,
+                    help="untimed graph replays before the timed ones (the GPU's clocks settle)")
     args = ap.parse_args(argv)
     import torch
     import tucuda_amd
@@ -81,23 +84,28 @@ def main(argv=None):
             else:
                 f = tucuda_amd.FIRFilter2().initialize(h, v)
 
-            def run():
-                for x, o in zip(ins, outs):
-                    if kind == "gauss6":
-                        f.convolve6(x, o)
-                    else:
-                        f.convolve(x, o[0])
+            def run(rounds):
+                for _ in range(rounds):
+                    for x, o in zip(ins, outs):
+                        if kind == "gauss6":
+                            f.convolve6(x, o)
+                        else:
+                            f.convolve(x, o[0])
 
-            run()
+            run(1)
             torch.cuda.synchronize()
+            # 48 launches per graph (the 4 buffer sets 12 times): the cost of
+            # starting a replay then adds little per launch
+            rounds = 12
             graph = torch.cuda.CUDAGraph()
             st = torch.cuda.Stream(dev)
             st.wait_stream(torch.cuda.current_stream(dev))
             with torch.cuda.stream(st):
                 with torch.cuda.graph(graph, stream=st):
-                    run()
+                    run(rounds)
             torch.cuda.current_stream(dev).wait_stream(st)
-            graph.replay()
+            for _ in range(max(1, args.warmup)):
+                graph.replay()
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             rs = torch.cuda.current_stream(dev)
@@ -106,7 +114,7 @@ def main(argv=None):
                 graph.replay()
             e1.record(rs)
             torch.cuda.synchronize()
-            us = e0.elapsed_time(e1) * 1000 / (args.reps * len(outs))
+            us = e0.elapsed_time(e1) * 1000 / (args.reps * rounds * len(outs))
             entry[fam] = {"us": round(us, 2), "gb_s": round(nbytes / us / 1e3, 1)}
             del graph
         os.environ.pop("TUCU_FIR_PATH", None)
