@@ -1,10 +1,10 @@
 """ICIA error-moment measurement.
 
-    python -m tucuda_amd.bench_icia [--reps 40]
+    python -m tucuda_amd.bench_icia [--reps 40] [--warmup 20]
 
 Workloads: a 1920x1080 float32 image and a 3840x2160 u8 image (seeded
-synthetic edges), sigma 2 and 3.  Times, from a captured HIP graph over 4
-rotating input buffers, (a) tucu_icia_moment -- diffH and diffV in one pass
+synthetic edges), sigma 2 and 3.  Times, from a captured HIP graph of 48
+calls over 4 rotating input buffers, after `--warmup` untimed replays, (a) tucu_icia_moment -- diffH and diffV in one pass
 that never writes them, reduced to the 21 (affine) or 36 (projective)
 moments -- and (b) the fused diffH + diffV launch that writes both gradient
 images, the first half of the reference's ICIA step (ICIA.h:344-350).
@@ -24,6 +24,8 @@ import numpy as np
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=40, help="graph replays timed per workload")
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed graph replays before the timed ones (the GPU's clocks settle)")
     args = ap.parse_args(argv)
     import torch
     import tucuda_amd
@@ -56,14 +58,14 @@ def main(argv=None):
                         assert lib.tucu_icia_moment(l0.ctypes.data_as(fp), l0.size, l1.ctypes.data_as(fp), l1.size,
                                                     ctypes.byref(src), m, ctypes.c_void_p(out.data_ptr()),
                                                     ctypes.c_void_p(ws.data_ptr()), ctypes.byref(n), sh) == 0
-                entry[model] = round(_graph_us(torch, dev, st, run, args.reps) / len(ins), 2)
+                entry[model] = round(_graph_us(torch, dev, st, run, args.reps, args.warmup) / len(ins), 2)
             g = tucuda_amd.FIRGaussianConvolver2(sigma)
             grads = [(torch.empty_like(x, dtype=torch.float32), torch.empty_like(x, dtype=torch.float32)) for x in ins]
 
             def run_grad(sh):
                 for x, (eh, ev) in zip(ins, grads):
                     g.convolve6(x, {"diffH": eh, "diffV": ev})
-            entry["diffH_diffV_written"] = round(_graph_us(torch, dev, st, run_grad, args.reps)
+            entry["diffH_diffV_written"] = round(_graph_us(torch, dev, st, run_grad, args.reps, args.warmup)
                                                  / len(ins), 2)
             nbytes = img.numel() * img.element_size()
             entry["moment_gb_s"] = round(nbytes / entry["affine"] / 1e3, 1)
@@ -72,8 +74,10 @@ def main(argv=None):
     print(json.dumps(res), flush=True)
 
 
-def _graph_us(torch, dev, st, run, reps):
-    """Microseconds per replay of a graph holding run()'s launches."""
+def _graph_us(torch, dev, st, run, reps, warmup, rounds=12):
+    """Microseconds per run() from a graph holding `rounds` run()'s (so that
+    the cost of starting a replay adds little per call), after `warmup`
+    untimed replays."""
     sh = ctypes.c_void_p(st.cuda_stream)
     st.wait_stream(torch.cuda.current_stream(dev))
     with torch.cuda.stream(st):
@@ -82,9 +86,11 @@ def _graph_us(torch, dev, st, run, reps):
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.stream(st):
         with torch.cuda.graph(graph, stream=st):
-            run(sh)
+            for _ in range(rounds):
+                run(sh)
     torch.cuda.synchronize()
-    graph.replay()
+    for _ in range(max(1, warmup)):
+        graph.replay()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     rs = torch.cuda.current_stream(dev)
@@ -93,7 +99,7 @@ def _graph_us(torch, dev, st, run, reps):
         graph.replay()
     e1.record(rs)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) * 1000 / reps
+    return e0.elapsed_time(e1) * 1000 / (reps * rounds)
 
 
 if __name__ == "__main__":

@@ -14,7 +14,8 @@ Workloads (seeded synthetic images, inputs resident in HBM):
   * one fused step on 1920x1080 float32 (sigma 1 and 3), 3840x2160 float32
     (sigma 3) and 3840x2160 u_char (sigma 2) images; algorithmic bytes: the
     input once plus the quarter-size float32 output once.
-Launches are replayed from a captured HIP graph over 4 rotating buffer sets.
+Launches are replayed from a captured HIP graph that goes 12 times over 4
+rotating buffer sets, after 40 untimed replays.
 Algorithmic bytes of a subsample: the input rows the operator reads (every
 second row for the 1-row operators, all rows otherwise) once, plus the
 nrow/2 x ncol/2 output once.  Prints one JSON line: microseconds per launch
@@ -30,7 +31,10 @@ HBM_PEAK_GB_S = 8000.0
 ONE_ROW_OPS = ("identity1x1", "diffH1x3", "diffHH1x3")
 
 
-def _graph_us(run, nsets, dev, reps=20):
+def _graph_us(run, nsets, dev, reps=20, rounds=12, warmup=40):
+    """Microseconds per launch (per set) from a graph holding `rounds` run()'s,
+    so that the cost of starting a replay adds little per launch, after
+    `warmup` untimed replays (the GPU's clocks settle)."""
     import torch
     run()
     torch.cuda.synchronize()
@@ -39,9 +43,10 @@ def _graph_us(run, nsets, dev, reps=20):
     st.wait_stream(torch.cuda.current_stream(dev))
     with torch.cuda.stream(st):
         with torch.cuda.graph(graph, stream=st):
-            run()
+            for _ in range(rounds):
+                run()
     torch.cuda.current_stream(dev).wait_stream(st)
-    for _ in range(3):
+    for _ in range(warmup):
         graph.replay()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -51,7 +56,7 @@ def _graph_us(run, nsets, dev, reps=20):
         graph.replay()
     e1.record(rs)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) * 1000 / (reps * nsets)
+    return e0.elapsed_time(e1) * 1000 / (reps * rounds * nsets)
 
 
 def subsample_bytes(nrow, ncol, in_es, out_es, op):
