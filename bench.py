@@ -465,7 +465,7 @@ def main():
                         "smooth+diffH+diffV+diffHH+diffHV+diffVV fused in one launch",
             "nrow": nrow, "ncol": ncol, "sigma": SIGMA, "outputs": 6,
             "buffer_sets": args.sets, "shift": True,
-            "launch": "eager" if args.eager else "hip-graph replay",
+            "launch": "eager" if args.eager else f"hip-graph replay, {per} launches per graph",
             "kernel": f"persistent, {args.persistent}-row tiles" if args.persistent else "one workgroup per tile",
         },
         "roofline": {
