@@ -127,8 +127,8 @@ int launch_fir_mfma(SepArgs& a, const float* vt, const float* ht, int rV, int rH
     return launch(g_fir_mfma_table[std::max(rV, rH) <= 8 ? 0 : 1][in_dt][out_dt], a, s);
 }
 
-// Workgroup shape of the single-image matrix-core kernel: TUCU_MFMA_ROWS
-// (16, 32 or 64 tile rows) overrides the default (32).
+// Workgroup shape of the single-image six-output matrix-core kernel:
+// TUCU_MFMA_ROWS (16, 32 or 64 tile rows) overrides the default (64).
 int mfma_wy_index()
 {
     static const int idx = [] {
@@ -901,7 +901,7 @@ int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, con
         cy0 = std::min(cy0, r1); cy1 = std::max(cy1, nrow - r1); cx0 = std::min(cx0, r0); cx1 = std::max(cx1, ncol - r0);
     }
     const int D = model, NM = D * (D + 1) / 2;
-    constexpr int WY = MFMA_WYS[MFMA_WY_DEFAULT];
+    constexpr int WY = MFMA_WYS[MFMA_WY_ONE];
     int64_t nblocks = 0;
     if (live) {
         a.tiles_y = (cy1 - cy0 + 16 * WY - 1) / (16 * WY);

@@ -123,7 +123,7 @@ template <int RC, class IN, int D>
 __global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))   // 3 spill
 icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
 {
-    constexpr int WY = MFMA_WYS[MFMA_WY_DEFAULT];
+    constexpr int WY = MFMA_WYS[MFMA_WY_ONE];
     constexpr int NM = icia_nm<D>();
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];

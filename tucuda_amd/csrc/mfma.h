@@ -1124,7 +1124,13 @@ hipError_t launch_batch_mfma(const SepArgs* jobs, const BatchTile* tiles, int nt
 constexpr int MFMA_RADII[2] = {8, 16};
 constexpr int NMWY = 3;
 constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
-constexpr int MFMA_WY_DEFAULT = 1;          // index: 2 x 2 strips, 32 x 128 pixels
+// Default shapes (indices into MFMA_WYS), measured: the six-output kernels
+// 1 x 4 strips, 64 x 64 pixels (1080p f32 / bf16 / f16 2.5-4 % faster than
+// 2 x 2, 3840 x 2160 3 %: the strips of a workgroup share their vertical
+// halo rows in L1); the one-output FIR / general-taps kernels and the ICIA
+// kernel 2 x 2, 32 x 128 (the 4096^2 u8 tile loop 2.5-5 % slower with 1 x 4).
+constexpr int MFMA_WY_DEFAULT = 2;
+constexpr int MFMA_WY_ONE = 1;
 
 extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 extern KernelEntry g_fir_mfma_table[2][4][4];       // MODE_TAPS, default shape: [radius class][in][out]
