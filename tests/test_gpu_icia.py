@@ -101,3 +101,25 @@ def test_icia_moment_small_image_and_workspace(dev):
     # a constant image has zero gradients everywhere: the moment is 0
     got = tucuda_amd.icia_moment(2.0, y, "projective")
     assert float(got.abs().max()) == 0.0
+
+
+def test_icia_moment_many_partials_deterministic(dev):
+    """A 4096 x 8192 u8 image: more workgroup partials than the reduction's
+    lanes take in one pass of loads (its loop runs more than once).  The
+    moment agrees with the oracle's moment of the gradients the fused launch
+    writes, and two calls give the same bits (a fixed summation order)."""
+    import torch
+    import tucuda_amd
+    sigma = 2.0
+    img = synth.u8_image(9, 4096, 8192)
+    x = to_dev(img, dev)
+    g = tucuda_amd.FIRGaussianConvolver2(sigma)
+    eH = torch.zeros(x.shape, dtype=torch.float32, device=dev)
+    eV = torch.zeros_like(eH)
+    g.convolve6(x, {"diffH": eH, "diffV": eV})
+    a = tucuda_amd.icia_moment(sigma, x, "projective")
+    b = tucuda_amd.icia_moment(sigma, x, "projective")
+    got = a.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), b.cpu().numpy().view(np.uint32))
+    want, absm = oracle.icia_moment(eH.cpu().numpy(), eV.cpu().numpy(), "projective")
+    np.testing.assert_array_less(np.abs(got.astype(np.float64) - want), 1e-5 * absm)

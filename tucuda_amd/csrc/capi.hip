@@ -901,7 +901,7 @@ int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, con
         cy0 = std::min(cy0, r1); cy1 = std::max(cy1, nrow - r1); cx0 = std::min(cx0, r0); cx1 = std::max(cx1, ncol - r0);
     }
     const int D = model, NM = D * (D + 1) / 2;
-    constexpr int WY = MFMA_WYS[MFMA_WY_ONE];
+    constexpr int WY = MFMA_WYS[MFMA_WY_ICIA];
     int64_t nblocks = 0;
     if (live) {
         a.tiles_y = (cy1 - cy0 + 16 * WY - 1) / (16 * WY);

@@ -20,8 +20,8 @@
 // they give into its FP32 moment sums.  A workgroup filters up to
 // ICIA_MAX_TILES tiles; its lanes' sums are added within each row of 16
 // lanes by DPP, and across rows and waves in FP64, into one partial per
-// workgroup; icia_reduce_kernel adds the partials in FP64 in a fixed order,
-// so the result does not depend on timing.
+// workgroup; icia_reduce_kernel adds the partials in FP64 in a fixed order
+// (one wave per moment), so the result does not depend on timing.
 #pragma once
 
 #include "mfma.h"
@@ -123,7 +123,7 @@ template <int RC, class IN, int D>
 __global__ void __launch_bounds__(64 * MWAVES) __attribute__((amdgpu_waves_per_eu(2)))   // 3 spill
 icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
 {
-    constexpr int WY = MFMA_WYS[MFMA_WY_ONE];
+    constexpr int WY = MFMA_WYS[MFMA_WY_ICIA];
     constexpr int NM = icia_nm<D>();
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
@@ -177,30 +177,35 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
 }
 
 // moment[k] = the sum of partial[k][b] over the nblocks workgroups, in FP64
-// in a fixed order (thread t adds b = t, t + 128, ..., then thread k adds
-// the 128 thread sums in order); 0 when nblocks is 0.  The partials are
-// stored moment-major, so that each of the loads here reads 128
-// consecutive doubles.
+// in a fixed order; 0 when nblocks is 0.  One wave per moment (workgroup k
+// sums moment k): lane l adds b = l, l + 64, l + 128, ... in order, eight
+// loads in flight at a time (the partials are stored moment-major, so each
+// load reads 64 consecutive doubles), then the 64 lane sums are added by a
+// butterfly (xor 32, 16, ..., 1), whose two partners form the same sum, so
+// every lane ends with the same value and the result does not depend on
+// timing.
 template <int NM>
-__global__ void __launch_bounds__(128) icia_reduce_kernel(const double* __restrict__ partial, int nblocks,
-                                                          float* __restrict__ moment)
+__global__ void __launch_bounds__(64) icia_reduce_kernel(const double* __restrict__ partial, int nblocks,
+                                                         float* __restrict__ moment)
 {
-    __shared__ double sh[NM][129];
-    const int t = int(threadIdx.x);
-    double acc[NM];
+    constexpr int UNROLL = 8;
+    const int lane = int(threadIdx.x);
+    const double* p = partial + int64_t(blockIdx.x) * nblocks;
+    double s = 0.0;
+    for (int b0 = 0; b0 < nblocks; b0 += 64 * UNROLL) {
+        double x[UNROLL];
 #pragma unroll
-    for (int k = 0; k < NM; ++k) acc[k] = 0.0;
-    for (int b = t; b < nblocks; b += 128)
+        for (int j = 0; j < UNROLL; ++j) {
+            const int b = b0 + 64 * j + lane;
+            x[j] = b < nblocks ? p[b] : 0.0;
+        }
 #pragma unroll
-        for (int k = 0; k < NM; ++k) acc[k] += partial[int64_t(k) * nblocks + b];
-#pragma unroll
-    for (int k = 0; k < NM; ++k) sh[k][t] = acc[k];
-    __syncthreads();
-    if (t < NM) {
-        double s = 0.0;
-        for (int i = 0; i < 128; ++i) s += sh[t][i];
-        moment[t] = float(s);
+        for (int j = 0; j < UNROLL; ++j)
+            if (b0 + 64 * j + lane < nblocks) s += x[j];
     }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+    if (lane == 0) moment[blockIdx.x] = float(s);
 }
 
 // Workgroups of one ICIA launch over ntiles tiles: the resident workgroups,
@@ -222,7 +227,7 @@ hipError_t launch_icia(const SepArgs& a, int ntiles, double* partial, float* mom
         grid = icia_grid(icia_mfma_kernel<RC, IN, D>, ntiles);
         hipLaunchKernelGGL((icia_mfma_kernel<RC, IN, D>), dim3(grid), dim3(64 * MWAVES), 0, s, a, ntiles, partial);
     }
-    hipLaunchKernelGGL(icia_reduce_kernel<icia_nm<D>()>, dim3(1), dim3(128), 0, s, partial, grid, moment);
+    hipLaunchKernelGGL(icia_reduce_kernel<icia_nm<D>()>, dim3(icia_nm<D>()), dim3(64), 0, s, partial, grid, moment);
     return hipGetLastError();
 }
 

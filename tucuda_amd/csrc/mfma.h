@@ -1127,10 +1127,13 @@ constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
 // Default shapes (indices into MFMA_WYS), measured: the six-output kernels
 // 1 x 4 strips, 64 x 64 pixels (1080p f32 / bf16 / f16 2.5-4 % faster than
 // 2 x 2, 3840 x 2160 3 %: the strips of a workgroup share their vertical
-// halo rows in L1); the one-output FIR / general-taps kernels and the ICIA
-// kernel 2 x 2, 32 x 128 (the 4096^2 u8 tile loop 2.5-5 % slower with 1 x 4).
+// halo rows in L1); the one-output FIR / general-taps kernels 2 x 2, 32 x
+// 128 (the 4096^2 u8 tile loop 2.5-5 % slower with 1 x 4); the ICIA pass 1 x
+// 4 (1080p f32 sigma 3 projective 8 %, 3840 x 2160 u8 1-3 % faster, 1080p
+// sigma 2 2 % slower).
 constexpr int MFMA_WY_DEFAULT = 2;
 constexpr int MFMA_WY_ONE = 1;
+constexpr int MFMA_WY_ICIA = 2;
 
 extern KernelEntry g_gauss_mfma_table[2][NMWY][4][4];
 extern KernelEntry g_fir_mfma_table[2][4][4];       // MODE_TAPS, default shape: [radius class][in][out]
