@@ -54,12 +54,12 @@ __device__ __forceinline__ float row_sum(float v)
     return v;
 }
 
-// mom[k] += the moments of this lane's 16 pixels (row y, columns x0 + 16 o
-// + 4 g + i of the accumulators gH, gV), zero outside each gradient's
+// mom[k * ms] += the moments of this lane's 16 pixels (row y, columns x0 +
+// 16 o + 4 g + i of the accumulators gH, gV), zero outside each gradient's
 // valid region.
 template <int D>
 __device__ __forceinline__ void icia_lane_moments(const SepArgs& a, int y0, int x0, const v4f32* gH,
-                                                  const v4f32* gV, float* mom)
+                                                  const v4f32* gV, float* mom, int ms)
 {
     constexpr int NU = D == 6 ? 3 : 5;          // powers of u: 0 .. 2 (affine), 0 .. 4
     const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
@@ -111,7 +111,7 @@ __device__ __forceinline__ void icia_lane_moments(const SepArgs& a, int y0, int 
                     const float t = A[P.x + Q.x][P.i + Q.i] * vp[P.j + Q.j];
                     s += P.sgn * Q.sgn > 0 ? t : -t;
                 }
-            mom[k] += s;
+            mom[k * ms] += s;
         }
 }
 
@@ -128,6 +128,16 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
     __shared__ MfmaTabs T;
     __shared__ v4f32 stage4[MWAVES * 16 * MOPITCH / 4];
     __shared__ float red[MWAVES][4][NM];
+    // Each lane's FP32 moment sums: in registers, or, for FP32 input at
+    // radius class 16 with the projective map, in LDS (moment k of thread t
+    // at momsh[k][t]), where in registers they stayed live through the strip
+    // code and it spilled 80 bytes per lane (1080p: 14.2 against 16.7 us per
+    // call; the LDS round trips made the other instances 4-14 % slower).
+    constexpr bool MLDS = RC == 16 && D == 8 && std::is_same_v<IN, float>;
+    constexpr int MS = MLDS ? 64 * MWAVES : 1;
+    __shared__ float momsh[MLDS ? NM * MS : 1];
+    float momr[MLDS ? 1 : NM];
+    float* const mom = MLDS ? momsh + threadIdx.x : momr;
     mfma_args_early(a);
     float* const stage = reinterpret_cast<float*>(stage4) + (threadIdx.x >> 6) * 16 * MOPITCH;
     const int grid = int(gridDim.x);
@@ -138,9 +148,8 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
     mfma_get_tabs<MODE_GAUSS>(a, T);
     __syncthreads();
-    float mom[NM];
 #pragma unroll
-    for (int k = 0; k < NM; ++k) mom[k] = 0.f;
+    for (int k = 0; k < NM; ++k) mom[k * MS] = 0.f;
     while (t < ntiles) {                            // uniform over the workgroup
         if (live) {
             v4f32 gH[MNB], gV[MNB];
@@ -153,7 +162,7 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
                         else        gV[o] = acc[o] * u;
                     }
                 }, a.mask);
-            icia_lane_moments<D>(a, y0, x0, gH, gV, mom);
+            icia_lane_moments<D>(a, y0, x0, gH, gV, mom, MS);
         }
         t += grid;
         live = t < ntiles && mfma_strip_pos<WY>(a, t, y0, x0);
@@ -162,7 +171,7 @@ icia_mfma_kernel(const SepArgs a, int ntiles, double* __restrict__ partial)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
-        const float s = row_sum(mom[k]);
+        const float s = row_sum(mom[k * MS]);
         if ((lane & 15) == 0) red[w][lane >> 4][k] = s;
     }
     __syncthreads();
