@@ -219,8 +219,9 @@ TUCU_API int tucu_gauss2_pyramid(const float* lobe0, int n0, const tucu_image* i
  * reference's freshly allocated gradient images hold zeros there).
  * `moment`: device memory for 21 or 36 floats.  Call first with workspace ==
  * NULL to get the workspace size in *workspace_bytes (as cub's first call,
- * ICIA.h:355-360), then with a device buffer at least that large.  A NaN or
- * an Inf in the image makes the moment non-finite, as the reference's sum. */
+ * ICIA.h:355-360), then with a device buffer at least that large; it must
+ * overlap neither the image nor `moment` (TUCU_ERR_INVALID).  A NaN or an
+ * Inf in the image makes the moment non-finite, as the reference's sum. */
 enum { TUCU_ICIA_AFFINE = 6, TUCU_ICIA_PROJECTIVE = 8 };
 TUCU_API int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, const tucu_image* in,
                               int model, float* moment, void* workspace, size_t* workspace_bytes, void* stream);

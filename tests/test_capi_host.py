@@ -167,3 +167,37 @@ def test_server_admission_rule():
     assert f(100 * M, 50 * G, G, 4 * G, 4 * G + 64 * M, 256 * M) == 0       # beyond the cap
     assert f(10 * M, 50 * G, G, 4 * G, 4 * G, 256 * M) == 0                 # a fixed pool
     assert f(0, 50 * G, G, 0, 0, 256 * M) == 0
+
+
+def test_icia_moment_argument_checks():
+    """Argument checks of tucu_icia_moment, which return before any device
+    work: unknown model, lobe lengths outside 1..16, the workspace size
+    query, a too-small workspace, and a workspace that overlaps the image or
+    the moment."""
+    import tucuda_amd
+    lib = _capi.load()
+    fp = ctypes.POINTER(ctypes.c_float)
+    l0, l1, _ = tucuda_amd.gauss_lobes(2.0)
+    p0, p1 = l0.ctypes.data_as(fp), l1.ctypes.data_as(fp)
+    img = 0x100000                                   # never dereferenced on these paths
+    src = _img(img, 512, 200, 300, _capi.TUCU_F32)
+    need = ctypes.c_size_t(0)
+
+    def call(model, moment, ws, nbytes, n0=l0.size, n1=l1.size):
+        size = ctypes.c_size_t(nbytes)
+        return lib.tucu_icia_moment(p0, n0, p1, n1, ctypes.byref(src), model, ctypes.c_void_p(moment),
+                                    ctypes.c_void_p(ws), ctypes.byref(size), None)
+
+    assert lib.tucu_icia_moment(p0, l0.size, p1, l1.size, ctypes.byref(src), 6, None, None,
+                                ctypes.byref(need), None) == _capi.TUCU_OK
+    assert need.value > 0
+    n = need.value
+    other = 0x40000000                               # far from the image
+    assert call(7, other, other + 4096, n) == _capi.TUCU_ERR_INVALID
+    assert call(6, other, other + 4096, n, n0=1) == _capi.TUCU_ERR_UNSUPPORTED_LOBE
+    assert call(6, other, other + 4096, n, n1=17) == _capi.TUCU_ERR_UNSUPPORTED_LOBE
+    assert call(6, other, other + 4096, n - 1) == _capi.TUCU_ERR_INVALID
+    assert call(6, other + 256, other, n) == _capi.TUCU_ERR_INVALID                  # moment inside the workspace
+    assert b"moment overlaps the workspace" in lib.tucu_last_error()
+    assert call(6, other, img + 199 * 512 * 4, n) == _capi.TUCU_ERR_INVALID          # workspace on the last row
+    assert b"workspace overlaps the input" in lib.tucu_last_error()

@@ -123,3 +123,34 @@ def test_icia_moment_many_partials_deterministic(dev):
     assert np.array_equal(got.view(np.uint32), b.cpu().numpy().view(np.uint32))
     want, absm = oracle.icia_moment(eH.cpu().numpy(), eV.cpu().numpy(), "projective")
     np.testing.assert_array_less(np.abs(got.astype(np.float64) - want), 1e-5 * absm)
+
+
+def test_icia_moment_graph_replay(dev):
+    """tucu_icia_moment captured in a HIP graph (the moment pass and the
+    reduction; the workspace and the moment come from the graph's pool):
+    every replay gives the bits of the eager call on the image the input
+    buffer holds at that replay, with eager calls between the replays."""
+    import torch
+    import tucuda_amd
+    sigma = 3.0
+    imgs = [synth.edge_image(20 + i, 1080, 1920) for i in range(3)]
+    x = to_dev(imgs[0], dev)
+    refs = []
+    for img in imgs:                                  # eager; uploads the operand tables
+        refs.append(tucuda_amd.icia_moment(sigma, to_dev(img, dev), "projective").cpu().numpy())
+    st = torch.cuda.Stream(dev)
+    st.wait_stream(torch.cuda.current_stream(dev))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(st):
+        with torch.cuda.graph(graph, stream=st):
+            m = tucuda_amd.icia_moment(sigma, x, "projective")
+    torch.cuda.current_stream(dev).wait_stream(st)
+    torch.cuda.synchronize()
+    for rep in range(6):
+        i = rep % 3
+        x.copy_(to_dev(imgs[i], dev))
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(m.cpu().numpy().view(np.uint32), refs[i].view(np.uint32)), rep
+        tucuda_amd.icia_moment(2.0, to_dev(imgs[(i + 1) % 3], dev), "affine")   # eager work between replays
+        torch.cuda.synchronize()

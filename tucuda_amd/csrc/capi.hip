@@ -913,6 +913,15 @@ int tucu_icia_moment(const float* lobe0, int n0, const float* lobe1, int n1, con
     if (*workspace_bytes < need) return fail(TUCU_ERR_INVALID, "workspace smaller than tucu_icia_moment asked for");
     if (!moment) return fail(TUCU_ERR_INVALID, "null moment");
     if (live && !in->data) return fail(TUCU_ERR_INVALID, "input: null data");
+    {
+        // The moment pass writes its partials while other workgroups still
+        // read the input, and the reduction writes the moments while others
+        // still read partials: the workspace may overlap neither.
+        const uintptr_t w0 = reinterpret_cast<uintptr_t>(workspace), m0 = reinterpret_cast<uintptr_t>(moment);
+        const Span ws{w0, w0 + need}, ms{m0, m0 + size_t(NM) * sizeof(float)};
+        if (overlap(ws, ms)) return fail(TUCU_ERR_INVALID, "the moment overlaps the workspace");
+        if (live && overlap(ws, input_span(*in))) return fail(TUCU_ERR_INVALID, "the workspace overlaps the input");
+    }
     if (live) {
         const int64_t es = int64_t(dtype_size(in->dtype));
         const int64_t stride_b = in->stride * es;
