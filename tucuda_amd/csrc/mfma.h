@@ -980,11 +980,14 @@ sepfir_mfma_loop_kernel(const SepArgs a, int ntiles)
     if (live) mfma_load<RC, IN>(a, y0, x0, raw);    // in flight while the tables are built
     mfma_get_tabs<MODE>(a, T);
     __syncthreads();
-    // The six-output strip code takes its lane index afresh in each tile:
-    // left to itself the compiler computes the lane-derived offsets once,
-    // before the loop, and holds them through it (3840 x 2160 six outputs
-    // 1 % faster; the one-output loop 2.5-6 % slower, so it keeps them).
-    constexpr bool FRESH = MODE == MODE_GAUSS;
+    // The six-output strip code for float input takes its lane index afresh
+    // in each tile: left to itself the compiler computes the lane-derived
+    // offsets once, before the loop, and holds them through it (3840 x 2160
+    // six outputs 1 % faster).  The one-output loop ran 2.5-6 % slower so,
+    // and u8 input at radius class 8 3.5 % slower (its registers then fit
+    // three waves per SIMD, and the grid, the resident workgroups, grew by
+    // half): they keep the hoisted offsets.
+    constexpr bool FRESH = MODE == MODE_GAUSS && sizeof(IN) == 4;
     while (t < hi) {                                // uniform over the workgroup
         if (live) mfma_compute<RC, IN, OUT, MODE, CPOL_NT, FRESH>(a, y0, x0, raw, T, stage, [] {});
         t += step;
