@@ -154,3 +154,25 @@ def test_icia_moment_graph_replay(dev):
         assert np.array_equal(m.cpu().numpy().view(np.uint32), refs[i].view(np.uint32)), rep
         tucuda_amd.icia_moment(2.0, to_dev(imgs[(i + 1) % 3], dev), "affine")   # eager work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("model", ["affine", "projective"])
+@pytest.mark.parametrize("sigma", [2.0, 3.0])
+@pytest.mark.parametrize("ik", ["u8", "f16", "bf16"])
+def test_icia_moment_narrow_inputs(dev, ik, sigma, model):
+    """u8, float16 and bfloat16 images, both radius classes and both maps,
+    against the oracle's moment of the same values as float32 (16-bit
+    values are exact in float32)."""
+    import tucuda_amd
+    img = synth.edge_image(30, 257, 331)
+    if ik == "u8":
+        img = dev_img = np.clip(img, 0, 255).astype(np.uint8)
+    elif ik == "f16":
+        dev_img = img.astype(np.float16)
+        img = dev_img.astype(np.float32)
+    else:
+        dev_img = (img.view(np.uint32) >> 16).astype(np.uint16)      # truncation: exact bf16 values
+        img = (dev_img.astype(np.uint32) << 16).view(np.float32)
+    got = tucuda_amd.icia_moment(sigma, to_dev(dev_img, dev), model).cpu().numpy().astype(np.float64)
+    want, absm = _oracle_moment(img, sigma, model)
+    np.testing.assert_array_less(np.abs(got - want), 5e-5 * absm + 1e-30)
