@@ -121,6 +121,21 @@ def kernel_source_hash() -> str:
     return h.hexdigest()
 
 
+def kernel_label() -> str:
+    """The single-image kernel the library picks for this run: the
+    matrix-core strip kernel (tile rows from TUCU_MFMA_ROWS, default 64),
+    one workgroup per tile while the tiles fit the resident workgroups
+    (1080p: 510 tiles), else the resident workgroups looping over the
+    tiles; or the direct FP32 vector kernel under TUCU_GAUSS_PATH=valu."""
+    if os.environ.get("TUCU_GAUSS_PATH") == "valu":
+        return "direct FP32 vector kernel"
+    rows = os.environ.get("TUCU_MFMA_ROWS", "64")
+    rows = rows if rows in ("16", "32", "64") else "64"
+    cols = {"16": 256, "32": 128, "64": 64}[rows]
+    return (f"matrix-core strips, {rows} x {cols} tiles: one workgroup per tile while the tiles fit "
+            "the resident workgroups, else the resident workgroups looping over the tiles")
+
+
 def recorded_traffic():
     """(bytes per launch, stale) from the FETCH_SIZE / WRITE_SIZE record."""
     tfile = os.path.join(ROOT, "profiles", "gauss6_1080p_f32_traffic.json")
@@ -466,7 +481,7 @@ def main():
             "nrow": nrow, "ncol": ncol, "sigma": SIGMA, "outputs": 6,
             "buffer_sets": args.sets, "shift": True,
             "launch": "eager" if args.eager else f"hip-graph replay, {per} launches per graph",
-            "kernel": f"persistent, {args.persistent}-row tiles" if args.persistent else "one workgroup per tile",
+            "kernel": f"persistent, {args.persistent}-row tiles" if args.persistent else kernel_label(),
         },
         "roofline": {
             "bound": "hbm",
