@@ -1021,6 +1021,28 @@ hipError_t launch_sepfir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
     return hipGetLastError();
 }
 
+// One-output launches (MODE_TAPS), given the grid of the tile loop's shape
+// WYL: an image whose 16 WYR-row tiles fit the resident workgroups of the
+// one-round kernel gets one workgroup per such tile (1080p 17 x 17 lobes,
+// 1 x 4 strips: 7.0 against 7.3 us with 2 x 2); a larger one the tile loop
+// with WYL (the 4096^2 u8 launches 3-7 % slower with 1 x 4 than with 2 x 2).
+template <int RC, int WYR, int WYL, class IN, class OUT>
+hipError_t launch_fir_mfma(const SepArgs& a, int nblocks, hipStream_t s)
+{
+    static const int resident = mfma_resident(sepfir_mfma_kernel<RC, WYR, IN, OUT, MODE_TAPS>);
+    const int ty = (a.cy1 - a.cy0 + 16 * WYR - 1) / (16 * WYR);
+    const int tx = (a.cx1 - a.cx0 + MSTRIP_W * (MWAVES / WYR) - 1) / (MSTRIP_W * (MWAVES / WYR));
+    const int64_t n = int64_t(ty) * tx;
+    if (n > 0 && n <= resident) {
+        SepArgs r = a;
+        r.tiles_y = ty;
+        hipLaunchKernelGGL((sepfir_mfma_kernel<RC, WYR, IN, OUT, MODE_TAPS>), dim3(int(n)), dim3(64 * MWAVES), 0, s,
+                           r);
+        return hipGetLastError();
+    }
+    return launch_sepfir_mfma<RC, WYL, IN, OUT, MODE_TAPS>(a, nblocks, s);
+}
+
 // Matrix-core variant of batch_kernel: the same tile queue; the tap
 // tables in LDS are rebuilt when the job changes (tiles of a job are
 // contiguous in the list).
@@ -1140,10 +1162,10 @@ constexpr int MFMA_WYS[NMWY] = {1, 2, 4};
 // Default shapes (indices into MFMA_WYS), measured: the six-output kernels
 // 1 x 4 strips, 64 x 64 pixels (1080p f32 / bf16 / f16 2.5-4 % faster than
 // 2 x 2, 3840 x 2160 3 %: the strips of a workgroup share their vertical
-// halo rows in L1); the one-output FIR / general-taps kernels 2 x 2, 32 x
-// 128 (the 4096^2 u8 tile loop 2.5-5 % slower with 1 x 4); the ICIA pass 1 x
-// 4 (1080p f32 sigma 3 projective 8 %, 3840 x 2160 u8 1-3 % faster, 1080p
-// sigma 2 2 % slower).
+// halo rows in L1); the one-output FIR / general-taps tile loop 2 x 2, 32 x
+// 128 (the 4096^2 u8 launches 2.5-7 % slower with 1 x 4), its one-round
+// launches 1 x 4 (launch_fir_mfma); the ICIA pass 1 x 4 (1080p f32 sigma 3
+// projective 8 %, 3840 x 2160 u8 1-3 % faster, 1080p sigma 2 2 % slower).
 constexpr int MFMA_WY_DEFAULT = 2;
 constexpr int MFMA_WY_ONE = 1;
 constexpr int MFMA_WY_ICIA = 2;
